@@ -1,0 +1,342 @@
+"""The oracle: CPU restatement of realflare's render path (test infrastructure).
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` import this module, as the checker (or the timed CPU baseline);
+the render path in ``realflare_amd`` never does.
+
+* The OpenCL kernels are restated in C (``rf_oracle_impl.h``, built in float
+  and double into ``_build/librforacle.so`` by the Makefile) and wrapped here
+  with ctypes.
+* The host-side numpy transforms are restated here in numpy, in complex128
+  like the reference: the starburst power spectrum
+  (``realflare/api/tasks/starburst.py:45-81``) and the ghost texture's
+  fractional Fourier transform (``realflare/api/tasks/ghost.py:13-36``,
+  ``realflare/utils/frft.py:8-62``).
+
+Pinning: every function here is checked by ``tests/test_oracle*.py`` against
+the reference test vectors, closed forms, invariants, brute force or golden
+fixtures; the header of each test names which.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from functools import lru_cache
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, '_build', 'librforacle.so')
+
+LAMBDA_MIN, LAMBDA_MAX = 390, 730
+LAMBDA_MID = (LAMBDA_MIN + LAMBDA_MAX) / 2
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(['make', '-s', '-C', HERE], check=True)
+    return LIB_PATH
+
+
+@lru_cache(1)
+def lib() -> ctypes.CDLL:
+    if not os.path.exists(LIB_PATH):
+        build()
+    return ctypes.CDLL(LIB_PATH)
+
+
+def threads() -> int:
+    return int(lib().rfo_threads())
+
+
+def _real(precision: str):
+    return np.float32 if precision == 'f32' else np.float64
+
+
+def ray_dtype(precision: str = 'f32') -> np.dtype:
+    r = _real(precision)
+    return np.dtype(
+        [('pos', r, 3), ('dir', r, 3), ('pos_apt', r, 2), ('rrel', r), ('reflectance', r)]
+    )
+
+
+def inter_dtype(precision: str = 'f32') -> np.dtype:
+    r = _real(precision)
+    fields = [('pos', r, 3), ('normal', r, 3), ('incident', r), ('hit', np.int32)]
+    dt = np.dtype(fields, align=True)
+    return dt
+
+
+def vertex_dtype(precision: str = 'f32') -> np.dtype:
+    r = _real(precision)
+    return np.dtype(
+        [('pos', r, 2), ('uv', r, 2), ('rrel', r), ('reflectance', r), ('intensity', r)]
+    )
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _fn(name: str, precision: str):
+    return getattr(lib(), f'{name}_{precision}')
+
+
+def trace(
+    lens: np.ndarray,
+    paths,
+    wavelengths,
+    aperture_index: int,
+    coating_min_ior: float,
+    grid_count: int,
+    grid_length: float,
+    direction,
+    precision: str = 'f32',
+    intersections: bool = False,
+):
+    """``raytrace`` over the whole (P, W, R) NDRange (raytracing.cl:218-359)."""
+    lens = np.ascontiguousarray(lens)
+    assert lens.dtype.itemsize == 64
+    paths = np.ascontiguousarray(np.asarray(paths, np.int32).reshape(-1, 2))
+    wavelengths = np.ascontiguousarray(np.asarray(wavelengths, np.int32))
+    P, W, R = len(paths), len(wavelengths), grid_count * grid_count
+    out = np.zeros((P, W, R), ray_dtype(precision))
+    direction = np.ascontiguousarray(np.asarray(direction, np.float64).reshape(4))
+    inters = None
+    inters_count = 3 * len(lens) - 1
+    if intersections:
+        inters = np.zeros((P, W, R, inters_count), inter_dtype(precision))
+    f = _fn('rfo_trace', precision)
+    f.restype = ctypes.c_int
+    f(
+        _p(lens), ctypes.c_int(len(lens)), _p(paths), ctypes.c_int(P), _p(wavelengths),
+        ctypes.c_int(W), ctypes.c_int(aperture_index), ctypes.c_double(coating_min_ior),
+        ctypes.c_int(grid_count), ctypes.c_double(grid_length), _p(direction), _p(out),
+        _p(inters) if inters is not None else ctypes.c_void_p(0), ctypes.c_int(inters_count),
+    )
+    if intersections:
+        return out, inters
+    return out
+
+
+def prim_shader(rays: np.ndarray, grid_count: int, area_orig: float, min_area: float,
+                precision: str = 'f32'):
+    """``prim_shader`` (rasterizing.cl:233-320): bounds[P][Q][4], intensities[P][Q][W]."""
+    P, W, R = rays.shape
+    Q = (grid_count - 1) ** 2
+    rays = np.ascontiguousarray(rays)
+    bounds = np.zeros((P, Q, 4), _real(precision))
+    intens = np.zeros((P, Q, W), _real(precision))
+    _fn('rfo_prim_shader', precision)(
+        _p(rays), ctypes.c_int(P), ctypes.c_int(W), ctypes.c_int(grid_count),
+        ctypes.c_double(area_orig), ctypes.c_double(min_area), _p(bounds), _p(intens),
+    )
+    return bounds, intens
+
+
+def vertex_shader(rays, intensities, grid_count, screen_transform, resolution,
+                  precision: str = 'f32'):
+    """``vertex_shader`` (rasterizing.cl:322-386): vertexes[P][R][W]."""
+    P, W, R = rays.shape
+    out = np.zeros((P, R, W), vertex_dtype(precision))
+    _fn('rfo_vertex_shader', precision)(
+        _p(np.ascontiguousarray(rays)), _p(np.ascontiguousarray(intensities)), ctypes.c_int(P),
+        ctypes.c_int(W), ctypes.c_int(grid_count), ctypes.c_double(screen_transform),
+        ctypes.c_int(resolution[0]), ctypes.c_int(resolution[1]), _p(out),
+    )
+    return out
+
+
+def rasterize(vertexes, bounds, wavelength_sub_count, grid_count, anti_aliasing, intensity,
+              ghost_scale, ghost, spectrum, screen_transform, bin_size, resolution,
+              rows=None, precision: str = 'f32', binned: bool = True):
+    """``binner`` + ``rasterizer`` (rasterizing.cl:388-663) into a cleared image.
+
+    ``intensity`` is the value the host passes to the kernel (light intensity
+    times 1e3, rasterizing.py:415).  ``rows`` = (y0, y1) restricts the work to
+    a band of unflipped rows; pixels outside stay 0.
+    """
+    P, R, W = vertexes.shape
+    width, height = resolution
+    ghost = np.ascontiguousarray(ghost, np.float32)
+    spectrum = np.ascontiguousarray(spectrum, np.float32)
+    out = np.zeros((height, width, 4), _real(precision))
+    y0, y1 = rows if rows is not None else (0, height)
+    common = (
+        _p(np.ascontiguousarray(vertexes)), _p(np.ascontiguousarray(bounds)), ctypes.c_int(P),
+        ctypes.c_int(W), ctypes.c_int(wavelength_sub_count), ctypes.c_int(grid_count),
+        ctypes.c_int(anti_aliasing), ctypes.c_double(intensity), ctypes.c_double(ghost_scale),
+        _p(ghost), ctypes.c_int(ghost.shape[1]), ctypes.c_int(ghost.shape[0]), _p(spectrum),
+        ctypes.c_int(spectrum.shape[0]), ctypes.c_double(screen_transform),
+        ctypes.c_int(bin_size), ctypes.c_int(width), ctypes.c_int(height),
+    )
+    if binned:
+        _fn('rfo_rasterize', precision)(*common, ctypes.c_int(y0), ctypes.c_int(y1), _p(out))
+    else:
+        _fn('rfo_rasterize_unbinned', precision)(*common, _p(out))
+    return out
+
+
+def aperture_shape(resolution, size, blades, rotation, roundness, softness,
+                   precision: str = 'f32'):
+    """``aperture_shape`` (aperture.cl:5-47); rotation in radians, softness as
+    passed to the kernel (the project value / 10, aperture.py:96)."""
+    w, h = resolution
+    out = np.zeros((h, w), _real(precision))
+    _fn('rfo_aperture_shape', precision)(
+        ctypes.c_int(w), ctypes.c_int(h), ctypes.c_double(size[0]), ctypes.c_double(size[1]),
+        ctypes.c_int(blades), ctypes.c_double(rotation), ctypes.c_double(roundness),
+        ctypes.c_double(softness), _p(out),
+    )
+    return out
+
+
+def aperture_grating(img, size, strength, density, length, width, softness,
+                     precision: str = 'f32'):
+    h, w = img.shape
+    img = np.ascontiguousarray(img, _real(precision)).copy()
+    _fn('rfo_aperture_grating', precision)(
+        ctypes.c_int(w), ctypes.c_int(h), ctypes.c_double(size[0]), ctypes.c_double(size[1]),
+        ctypes.c_double(strength), ctypes.c_double(density), ctypes.c_double(length),
+        ctypes.c_double(width), ctypes.c_double(softness), _p(img),
+    )
+    return img
+
+
+def aperture_scratches(img, size, strength, density, length, rotation, rotation_variation,
+                       width, parallax, softness, precision: str = 'f32'):
+    h, w = img.shape
+    img = np.ascontiguousarray(img, _real(precision)).copy()
+    _fn('rfo_aperture_scratches', precision)(
+        ctypes.c_int(w), ctypes.c_int(h), ctypes.c_double(size[0]), ctypes.c_double(size[1]),
+        ctypes.c_double(strength), ctypes.c_double(density), ctypes.c_double(length),
+        ctypes.c_double(rotation), ctypes.c_double(rotation_variation), ctypes.c_double(width),
+        ctypes.c_double(parallax[0]), ctypes.c_double(parallax[1]), ctypes.c_double(softness),
+        _p(img),
+    )
+    return img
+
+
+def aperture_dust(img, size, strength, density, radius, parallax, softness,
+                  precision: str = 'f32'):
+    h, w = img.shape
+    img = np.ascontiguousarray(img, _real(precision)).copy()
+    _fn('rfo_aperture_dust', precision)(
+        ctypes.c_int(w), ctypes.c_int(h), ctypes.c_double(size[0]), ctypes.c_double(size[1]),
+        ctypes.c_double(strength), ctypes.c_double(density), ctypes.c_double(radius),
+        ctypes.c_double(parallax[0]), ctypes.c_double(parallax[1]), ctypes.c_double(softness),
+        _p(img),
+    )
+    return img
+
+
+def starburst(resolution, spectrum_img, light, samples, blur, rotation, rotation_weight,
+              vignetting, intensity, offset, size, precision: str = 'f32'):
+    """``starburst`` kernel (starburst.cl:1-87) with the host's argument
+    preparation already applied (starburst.py:96-106)."""
+    w, h = resolution
+    spectrum_img = np.ascontiguousarray(spectrum_img, np.float32)
+    light = np.ascontiguousarray(light, np.float32)
+    out = np.zeros((h, w, 4), _real(precision))
+    _fn('rfo_starburst', precision)(
+        ctypes.c_int(w), ctypes.c_int(h), _p(spectrum_img), ctypes.c_int(spectrum_img.shape[1]),
+        ctypes.c_int(spectrum_img.shape[0]), _p(light), ctypes.c_int(light.shape[0]),
+        ctypes.c_int(samples), ctypes.c_double(blur), ctypes.c_double(rotation),
+        ctypes.c_double(rotation_weight), ctypes.c_double(vignetting[0]),
+        ctypes.c_double(vignetting[1]), ctypes.c_double(intensity), ctypes.c_double(offset[0]),
+        ctypes.c_double(offset[1]), ctypes.c_double(size[0]), ctypes.c_double(size[1]), _p(out),
+    )
+    return out
+
+
+# ---------------------------------------------------------------------------
+# host-side numpy restatements
+# ---------------------------------------------------------------------------
+
+def cie_table() -> np.ndarray:
+    path = os.path.join(os.path.dirname(HERE), 'realflare_amd', 'resources', 'ciexyz_390_790.csv')
+    return np.loadtxt(path, delimiter=',', comments='#')
+
+
+def light_spectrum() -> np.ndarray:
+    """CIE XYZ rows with 390 <= lambda < 730 as RGBA (rasterizing.py:169-179)."""
+    t = cie_table()
+    sel = t[(t[:, 0] >= LAMBDA_MIN) & (t[:, 0] < LAMBDA_MAX)]
+    out = np.zeros((len(sel), 4), np.float32)
+    out[:, :3] = sel[:, 1:4]
+    return out
+
+
+def power_spectrum(aperture: np.ndarray, distance: float) -> np.ndarray:
+    """starburst.py:45-81: |fftshift(fft2(A * exp(i pi/(lambda d) (x^2+y^2))))|^2."""
+    wavelength = LAMBDA_MID * 1e-6
+    distance = max(1e-9, distance * 1e3)
+    h, w = aperture.shape[:2]
+    x = np.linspace(-1, 1, w)
+    y = np.linspace(-1, 1, h)
+    xv, yv = np.meshgrid(x, y)
+    exp = np.exp(1j * np.pi / (wavelength * distance) * (xv**2 + yv**2))
+    fft = np.fft.fftshift(np.fft.fft2(aperture * exp))
+    return np.float32(fft.real**2 + fft.imag**2)
+
+
+def _chirp(shape) -> np.ndarray:
+    """frft.py:8-17: i*pi*sum_k fftshift(n_k)^2 / N_k over an index grid."""
+    grids = np.mgrid[tuple(slice(-s // 2, s // 2) for s in shape)]
+    if len(shape) == 1:
+        grids = [grids[0]] if grids.ndim == 2 else [grids]
+    total = 0
+    for i, g in enumerate(grids):
+        total = total + np.fft.fftshift(g) ** 2 / g.shape[i]
+    return 1.0j * np.pi * total
+
+
+def _normalize(array: np.ndarray, alpha: float):
+    """frft.py:20-40: reduce alpha into [0.5, 1.5) with ortho (i)FFTs / flips."""
+    rel = alpha % 4
+    if rel < 0.5:
+        return np.fft.ifftn(array, norm='ortho'), rel + 1
+    if rel < 1.5:
+        return array, rel
+    if rel < 2.5:
+        return np.fft.fftn(array, norm='ortho'), rel - 1
+    if rel < 3.5:
+        return np.flip(array), rel - 2
+    return np.fft.ifftn(array, norm='ortho'), rel - 3
+
+
+def frft(array: np.ndarray, alpha: float) -> np.ndarray:
+    """frft.py:43-62: chirp-convolution discrete fractional Fourier transform."""
+    array, alpha = _normalize(np.asarray(array), alpha)
+    phi = alpha * np.pi / 2.0
+    cot = 1.0 / np.tan(phi)
+    sq = np.sqrt(1.0 + cot**2)
+    scale = np.sqrt(1.0 - 1.0j * cot) / np.sqrt(np.prod(array.shape))
+    arg = _chirp(array.shape)
+    chirp1 = np.exp(arg * (cot - sq))
+    chirp2 = np.exp(arg * sq)
+    f1 = np.fft.fftn(chirp2, norm='ortho')
+    f2 = np.fft.fftn(chirp1 * array, norm='ortho')
+    return scale * chirp1 * np.fft.ifftn(f1 * f2, norm='ortho')
+
+
+def ghost(aperture: np.ndarray, fstop: float, resolution) -> np.ndarray:
+    """ghost.py:13-36: |fftshift(frft(fftshift(A), alpha))| * sqrt(w*h)."""
+    alpha = 0.15 * (LAMBDA_MID / 400) * (fstop / 18)
+    spectrum = np.fft.fftshift(aperture)
+    spectrum = frft(spectrum, alpha)
+    spectrum = np.fft.fftshift(spectrum)
+    array = np.float32(np.abs(spectrum))
+    array *= np.sqrt(resolution[0] * resolution[1])
+    return array
+
+
+def to_reference_rays(rays: np.ndarray) -> np.ndarray:
+    """Repack oracle rays into the reference's 48-byte ``Ray`` records."""
+    from realflare_amd.api.dtypes import ray_dtype as ref_ray
+
+    out = np.zeros(rays.shape, ref_ray)
+    for name in ('pos', 'dir', 'pos_apt', 'rrel', 'reflectance'):
+        out[name] = rays[name]
+    return out

@@ -1,0 +1,119 @@
+// Shared definitions for the MI355X (gfx950) flare render kernels.
+//
+// Numerics follow the OpenCL kernels of realflare (realflare/api/tasks/cl),
+// with the behaviour that source leaves undefined fixed the same way the
+// oracle fixes it:
+//   * float -> int truncates toward zero, saturates, NaN -> 0 (rf_to_int);
+//   * integer edge functions wrap modulo 2^32;
+//   * float max/min are IEEE maxNum/minNum (fmaxf/fminf: NaN loses);
+//   * unwritten Vertex fields of rays that missed the sensor are 0;
+//   * image sampling is the OpenCL 1.2 bilinear formula in FP32, border 0;
+//   * unsuffixed double literals of the OpenCL source are applied in double.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/realflare_amd.h"
+
+#define RF_LMAX 64  // max lens elements (incl. stop and sensor) per lens
+
+namespace rf {
+
+// Records the error text for rf_last_error() and returns RF_ERR_*.
+int set_error(int code, const char* fmt, ...);
+int check_hip(hipError_t e, const char* what);
+
+#define RF_HIP(call)                                  \
+    do {                                              \
+        hipError_t e_ = (call);                       \
+        if (e_ != hipSuccess) return rf::check_hip(e_, #call); \
+    } while (0)
+
+#define RF_LAUNCH_CHECK(name)                                  \
+    do {                                                       \
+        hipError_t e_ = hipGetLastError();                     \
+        if (e_ != hipSuccess) return rf::check_hip(e_, name);  \
+    } while (0)
+
+struct f2 {
+    float x, y;
+};
+struct f3 {
+    float x, y, z;
+};
+
+__device__ __forceinline__ f3 mk3(float x, float y, float z) { return f3{x, y, z}; }
+__device__ __forceinline__ f3 add3(f3 a, f3 b) { return mk3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ f3 sub3(f3 a, f3 b) { return mk3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ f3 scl3(f3 a, float s) { return mk3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float len3(f3 a) { return sqrtf(dot3(a, a)); }
+__device__ __forceinline__ f3 norm3(f3 a) {
+    float l = len3(a);
+    return mk3(a.x / l, a.y / l, a.z / l);
+}
+
+__device__ __forceinline__ float clampn(float x, float a, float b) { return fminf(fmaxf(x, a), b); }
+
+__device__ __forceinline__ float smoothstepf(float e0, float e1, float x) {
+    float t = clampn((x - e0) / (e1 - e0), 0.0f, 1.0f);
+    return t * t * (3.0f - 2.0f * t);
+}
+
+__device__ __forceinline__ int rf_to_int(float f) {
+    if (!(f == f)) return 0;
+    if (f >= 2147483647.0f) return 2147483647;
+    if (f <= -2147483648.0f) return (int)(-2147483647 - 1);
+    return (int)f;
+}
+
+__device__ __forceinline__ int wsub(int a, int b) { return (int)((uint32_t)a - (uint32_t)b); }
+__device__ __forceinline__ int wmul(int a, int b) { return (int)((uint32_t)a * (uint32_t)b); }
+__device__ __forceinline__ int wadd(int a, int b) { return (int)((uint32_t)a + (uint32_t)b); }
+
+// OpenCL 1.2 (section 8.2) bilinear filter, normalized coordinates, one
+// channel at `stride` floats per texel.  edge=false: CLK_ADDRESS_CLAMP with
+// border 0; edge=true: CLK_ADDRESS_CLAMP_TO_EDGE.
+__device__ __forceinline__ float sample_bilinear(const float* __restrict__ tex, int w, int h,
+                                                 int stride, float s, float t, bool edge) {
+    float u = s * (float)w - 0.5f, v = t * (float)h - 0.5f;
+    float fu = floorf(u), fv = floorf(v);
+    int i0 = rf_to_int(fu), j0 = rf_to_int(fv);
+    int i1 = wadd(i0, 1), j1 = wadd(j0, 1);
+    float a = u - fu, b = v - fv;
+    float T[4];
+    int is[2] = {i0, i1}, js[2] = {j0, j1};
+#pragma unroll
+    for (int jj = 0; jj < 2; jj++) {
+#pragma unroll
+        for (int ii = 0; ii < 2; ii++) {
+            int i = is[ii], j = js[jj];
+            float val;
+            if (edge) {
+                i = i < 0 ? 0 : (i > w - 1 ? w - 1 : i);
+                j = j < 0 ? 0 : (j > h - 1 ? h - 1 : j);
+                val = tex[((long)j * w + i) * stride];
+            } else if (i < 0 || i >= w || j < 0 || j >= h) {
+                val = 0.0f;
+            } else {
+                val = tex[((long)j * w + i) * stride];
+            }
+            T[jj * 2 + ii] = val;
+        }
+    }
+    return (1.0f - a) * (1.0f - b) * T[0] + a * (1.0f - b) * T[1] + (1.0f - a) * b * T[2] +
+           a * b * T[3];
+}
+
+// color.cl:3-10, with the matrix applied in double as the source's double
+// literals ask for.
+__device__ __forceinline__ void xyz_to_ap1(float x, float y, float z, float* o) {
+    double dx = x, dy = y, dz = z;
+    o[0] = (float)(1.6410233797 * dx + -0.3248032942 * dy + -0.2364246952 * dz);
+    o[1] = (float)(-0.6636628587 * dx + 1.6153315917 * dy + 0.0167563477 * dz);
+    o[2] = (float)(0.0117218943 * dx + -0.0082844420 * dy + 0.9883948585 * dz);
+    o[3] = 0.0f;
+}
+
+}  // namespace rf
