@@ -88,6 +88,154 @@ int rf_trace_intersections(const rf_lens_element *lens, const int32_t *paths,
                            const int32_t *wavelengths, const rf_trace_params *prm, void *rays,
                            void *intersections, rf_stream_t stream);
 
+/* Settings of the rasterizer kernels (rasterizing.cl:233-663), as
+ * RasterizingTask.rasterize derives them (rasterizing.py:296-421). */
+typedef struct rf_raster_params {
+    int32_t path_count, wavelength_count, grid_count;
+    int32_t width, height;        /* render.resolution */
+    int32_t bin_size;             /* render.bin_size: which quads a pixel visits */
+    int32_t anti_aliasing;        /* 1, 2, 4 or 8 samples per pixel */
+    int32_t wavelength_sub_count; /* 1 when wavelength_count == 1 (py:390-392) */
+    float area_orig;              /* (grid_length / (grid_count - 1))^2 (py:152-156) */
+    float min_area;               /* lens.min_area * area_orig (py:329) */
+    float screen_transform;       /* width / half sensor diagonal (py:158-167) */
+    float intensity;              /* light.intensity * 1e3 (py:415) */
+    float ghost_scale;            /* 1 - lens.fstop / 32 (py:393) */
+    int32_t ghost_width, ghost_height;
+    int32_t spectrum_count;       /* texels of the light spectrum, RGBA each */
+    int32_t reserved_;
+} rf_raster_params;
+
+/*
+ * Per-quad intensities and bounds.  Replaces `prim_shader` (rasterizing.cl:233-320,
+ * host rasterizing.py:321-344).
+ *   planes      float[6][P*W*R] from rf_trace_planes
+ *   bounds      float4[P][Q]   (x0, y0, x1, y1) in sensor mm, x0 = NaN if culled
+ *   intensities float[P][Q][W] (entries the reference leaves unwritten are 0)
+ */
+int rf_prim_shader(const float *planes, const rf_raster_params *prm, float *bounds,
+                   float *intensities, rf_stream_t stream);
+
+/*
+ * Screen-space vertices.  Replaces `vertex_shader` (rasterizing.cl:322-386,
+ * host rasterizing.py:348-362).  vertexes: Vertex[P][R][W], 32-byte records
+ * (opencl.py:51-61); fields of rays that missed the sensor are 0 except
+ * reflectance = NaN.
+ */
+int rf_vertex_shader(const float *planes, const float *intensities, const rf_raster_params *prm,
+                     void *vertexes, rf_stream_t stream);
+
+/*
+ * Binning + rasterization.  Replaces `binner` and `rasterizer`
+ * (rasterizing.cl:388-663, host rasterizing.py:366-418, the image clear at
+ * py:263-266 included).  Every pixel sums the fragments of the quads the
+ * reference's binner lists for its bin, in the reference's order.
+ *   ghost           float[ghost_height][ghost_width] (GhostTask output)
+ *   light_spectrum  float[spectrum_count][4] (CIE XYZ, rasterizing.py:169-179)
+ *   image           float[height][width][4], fully written
+ * pair_capacity bounds the (8x8 tile, quad step) pairs a frame may bin; the
+ * workspace (rf_rasterize_workspace_size) grows with it.  If a frame needs
+ * more, nothing is drawn and rf_raster_status reports overflow and the count
+ * needed; call again with a larger capacity.
+ */
+size_t rf_rasterize_workspace_size(const rf_raster_params *prm, uint64_t pair_capacity);
+int rf_rasterize(const void *vertexes, const float *bounds, const float *ghost,
+                 const float *light_spectrum, const rf_raster_params *prm,
+                 uint64_t pair_capacity, void *workspace, size_t workspace_bytes, float *image,
+                 rf_stream_t stream);
+
+/*
+ * The whole RasterizingTask.run (rasterizing.py:423-439): prim_shader,
+ * vertex_shader, binner and rasterizer from trace planes to image, with all
+ * intermediates in the workspace (rf_flare_workspace_size).
+ */
+size_t rf_flare_workspace_size(const rf_raster_params *prm, uint64_t pair_capacity);
+int rf_flare(const float *planes, const float *ghost, const float *light_spectrum,
+             const rf_raster_params *prm, uint64_t pair_capacity, void *workspace,
+             size_t workspace_bytes, float *image, rf_stream_t stream);
+
+/*
+ * Reads back (synchronizing `stream`) the pair count the last rf_rasterize /
+ * rf_flare on this workspace needed and whether it overflowed the capacity.
+ * No reference counterpart: the reference sizes its bin queues on the host
+ * (rasterizing.py:190-208).
+ */
+int rf_raster_status(const void *workspace, uint64_t *required_pairs, int *overflow,
+                     rf_stream_t stream);
+
+/* Aperture settings as ApertureTask.aperture passes them to its kernels
+ * (aperture.py:80-153): angles in radians, softness / 10, widths and the
+ * dust radius * 0.1; a mask is applied when its strength is > 0. */
+typedef struct rf_aperture_params {
+    int32_t width, height;
+    float size[2];
+    int32_t blades;
+    float rotation, roundness, softness;
+    float grating_strength, grating_density, grating_length, grating_width, grating_softness;
+    float scratches_strength, scratches_density, scratches_length, scratches_rotation,
+        scratches_rotation_variation, scratches_width, scratches_parallax[2], scratches_softness;
+    float dust_strength, dust_density, dust_radius, dust_parallax[2], dust_softness;
+} rf_aperture_params;
+
+/*
+ * Aperture image float[height][width].  Replaces aperture_shape,
+ * aperture_grating, aperture_scratches and aperture_dust
+ * (aperture.cl:5-174) as ApertureTask.aperture runs them (aperture.py:63-186).
+ */
+int rf_aperture(const rf_aperture_params *prm, float *image, rf_stream_t stream);
+
+/*
+ * In-place 2-D FFT of interleaved complex64 data[height][width] (powers of
+ * two up to 4096), unnormalized, then multiplied by `scale`: numpy's fft2 is
+ * scale 1, ifft2 is inverse with 1/(w h), norm='ortho' is 1/sqrt(w h).
+ * Replaces the numpy.fft calls of starburst.py:71 and frft.py:27-60.
+ */
+size_t rf_fft_workspace_size(int width, int height);
+int rf_fft2(void *data, int width, int height, int inverse, float scale, void *workspace,
+            size_t workspace_bytes, rf_stream_t stream);
+
+/*
+ * Starburst power spectrum |fftshift(fft2(A exp(i pi/(lambda d)(x^2+y^2))))|^2
+ * as float[height][width].  Replaces StarburstTask.update_fourier_spectrum
+ * (starburst.py:45-81); `distance` is Flare.Starburst.distance (m).
+ */
+size_t rf_power_spectrum_workspace_size(int width, int height);
+int rf_power_spectrum(const float *aperture, int width, int height, float distance,
+                      void *workspace, size_t workspace_bytes, float *spectrum,
+                      rf_stream_t stream);
+
+/*
+ * Ghost texture |fftshift(frft(fftshift(A), alpha))| * sqrt(w h), alpha =
+ * 0.15 (560/400) (fstop/18).  Replaces GhostTask.ghost (ghost.py:13-36) and
+ * frft.frft (frft.py:43-62).
+ */
+size_t rf_ghost_workspace_size(int width, int height);
+int rf_ghost(const float *aperture, int width, int height, float fstop, void *workspace,
+             size_t workspace_bytes, float *ghost, rf_stream_t stream);
+
+/* Starburst kernel settings as StarburstTask.starburst prepares them
+ * (starburst.py:96-106, 131-141). */
+typedef struct rf_starburst_params {
+    int32_t width, height; /* render.resolution */
+    int32_t samples;
+    float blur;            /* Starburst.blur / 100 */
+    float rotation;        /* radians */
+    float rotation_weight;
+    float vignetting[2];   /* NaN, NaN when vignetting is off */
+    float intensity;       /* Starburst.intensity * 1e-6 */
+    float offset[2];       /* (light.x, -light.y) */
+    float size[2];         /* (scale.w, scale.h * width / height) */
+} rf_starburst_params;
+
+/*
+ * Starburst image float[height][width][4].  Replaces the `starburst` kernel
+ * (starburst.cl:1-87).  spectrum: rf_power_spectrum output;
+ * light_spectrum: float[spectrum_count][4] CIE XYZ.
+ */
+int rf_starburst(const float *spectrum, int spectrum_width, int spectrum_height,
+                 const float *light_spectrum, int spectrum_count,
+                 const rf_starburst_params *prm, float *image, rf_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

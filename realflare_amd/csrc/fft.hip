@@ -1,0 +1,364 @@
+// 2-D FFT, starburst power spectrum and ghost fractional Fourier transform.
+//
+// The reference runs these on the host in numpy complex128:
+//   * StarburstTask.update_fourier_spectrum (realflare/api/tasks/starburst.py:45-81):
+//     |fftshift(fft2(A * exp(i pi / (lambda d) (x^2 + y^2))))|^2;
+//   * GhostTask.ghost (realflare/api/tasks/ghost.py:13-36) with
+//     realflare/utils/frft.py:8-62: |fftshift(frft(fftshift(A), alpha))| * sqrt(w h).
+// Here they are complex64 on the device.  The FFT is a radix-2
+// decimation-in-time transform held entirely in LDS: a row pass (one
+// workgroup per row) and a column pass (a workgroup per group of adjacent
+// columns, so every global access reads contiguous runs of a row).  Twiddles
+// and chirp phases are evaluated in double and rounded once to float.
+#include <cmath>
+
+#include "primitives.h"
+
+namespace rf {
+
+static constexpr int FFT_THREADS = 256;
+static constexpr int FFT_LDS_BYTES = 65536;
+static constexpr int FFT_MAX_N = 4096;
+
+static int ilog2(int n) {
+    int l = 0;
+    while ((1 << l) < n) l++;
+    return l;
+}
+static bool pow2(int n) { return n >= 1 && (n & (n - 1)) == 0; }
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+#pragma clang fp contract(off)
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// tw[k] = exp(-2 pi i k / n), k < n/2
+__global__ void k_twiddles(float2* __restrict__ tw, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n / 2) return;
+    double s, c;
+    sincospi(-2.0 * (double)k / (double)n, &s, &c);
+    tw[k] = make_float2((float)c, (float)s);
+}
+
+// In-LDS radix-2 transform of `count` sequences of length n stored
+// contiguously at s[seq * n], already in bit-reversed order.
+__device__ __forceinline__ void lds_fft(float2* s, int n, int count, const float2* __restrict__ tw,
+                                        bool inverse) {
+#pragma clang fp contract(off)
+    const int total = count * (n >> 1);
+    for (int len = 2; len <= n; len <<= 1) {
+        const int half = len >> 1, tstride = n / len;
+        for (int b = threadIdx.x; b < total; b += blockDim.x) {
+            const int seq = b / (n >> 1), r = b % (n >> 1);
+            const int grp = r / half, k = r % half;
+            const int i0 = seq * n + grp * len + k, i1 = i0 + half;
+            float2 w = tw[k * tstride];
+            if (inverse) w.y = -w.y;
+            const float2 a = s[i0];
+            const float2 t = cmul(s[i1], w);
+            s[i0] = make_float2(a.x + t.x, a.y + t.y);
+            s[i1] = make_float2(a.x - t.x, a.y - t.y);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_rows(float2* __restrict__ data, int n, int logn,
+                                                           const float2* __restrict__ tw,
+                                                           int inverse, float scale) {
+    extern __shared__ float2 s[];
+    float2* row = data + (long)blockIdx.x * n;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const int r = logn ? (int)(__brev((unsigned)i) >> (32 - logn)) : 0;
+        s[r] = row[i];
+    }
+    __syncthreads();
+    lds_fft(s, n, 1, tw, inverse != 0);
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+        row[i] = make_float2(s[i].x * scale, s[i].y * scale);
+}
+
+// columns [c0, c0 + cols) of a (h x w) array; LDS layout s[col * h + row]
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_cols(float2* __restrict__ data, int w, int h,
+                                                           int logh, int cols,
+                                                           const float2* __restrict__ tw,
+                                                           int inverse, float scale) {
+    extern __shared__ float2 s[];
+    const int c0 = blockIdx.x * cols;
+    const int total = cols * h;
+    for (int i = threadIdx.x; i < total; i += blockDim.x) {
+        const int row = i / cols, c = i % cols;
+        const int r = logh ? (int)(__brev((unsigned)row) >> (32 - logh)) : 0;
+        s[c * h + r] = data[(long)row * w + c0 + c];
+    }
+    __syncthreads();
+    lds_fft(s, h, cols, tw, inverse != 0);
+    for (int i = threadIdx.x; i < total; i += blockDim.x) {
+        const int row = i / cols, c = i % cols;
+        const float2 v = s[c * h + row];
+        data[(long)row * w + c0 + c] = make_float2(v.x * scale, v.y * scale);
+    }
+}
+
+struct FftPlan {
+    int w, h, logw, logh, cols;
+    float2 *tw_w, *tw_h;
+};
+
+size_t fft_workspace_bytes(int w, int h) {
+    return align_up((size_t)(w / 2 + 1) * sizeof(float2), 256) +
+           align_up((size_t)(h / 2 + 1) * sizeof(float2), 256);
+}
+
+static int fft_plan(int w, int h, void* ws, hipStream_t s, FftPlan* p) {
+    if (!pow2(w) || !pow2(h) || w > FFT_MAX_N || h > FFT_MAX_N)
+        return set_error(RF_ERR_INVALID, "fft: sizes must be powers of two <= %d (got %dx%d)",
+                         FFT_MAX_N, w, h);
+    p->w = w;
+    p->h = h;
+    p->logw = ilog2(w);
+    p->logh = ilog2(h);
+    int cols = FFT_LDS_BYTES / (int)(h * sizeof(float2));
+    if (cols > 16) cols = 16;
+    if (cols > w) cols = w;
+    p->cols = cols < 1 ? 1 : cols;
+    p->tw_w = (float2*)ws;
+    p->tw_h = (float2*)((char*)ws + align_up((size_t)(w / 2 + 1) * sizeof(float2), 256));
+    if (w >= 2) hipLaunchKernelGGL(k_twiddles, dim3((w / 2 + 255) / 256), dim3(256), 0, s, p->tw_w, w);
+    if (h >= 2) hipLaunchKernelGGL(k_twiddles, dim3((h / 2 + 255) / 256), dim3(256), 0, s, p->tw_h, h);
+    RF_LAUNCH_CHECK("fft twiddles");
+    return RF_OK;
+}
+
+// unnormalized transform times `scale` (numpy: forward scale 1, inverse 1/N)
+static int fft_run(const FftPlan& p, float2* data, bool inverse, float scale, hipStream_t s) {
+    hipLaunchKernelGGL(k_fft_rows, dim3(p.h), dim3(FFT_THREADS), p.w * sizeof(float2), s, data, p.w,
+                       p.logw, p.tw_w, (int)inverse, scale);
+    hipLaunchKernelGGL(k_fft_cols, dim3(p.w / p.cols), dim3(FFT_THREADS),
+                       p.cols * p.h * sizeof(float2), s, data, p.w, p.h, p.logh, p.cols, p.tw_h,
+                       (int)inverse, 1.0f);
+    RF_LAUNCH_CHECK("fft");
+    return RF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// starburst power spectrum
+
+// c = A * exp(i pi / (lambda d) (x^2 + y^2)) on linspace(-1, 1) grids
+__global__ __launch_bounds__(256) void k_starburst_field(const float* __restrict__ ap, int w, int h,
+                                                         double k, float2* __restrict__ out) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long)w * h) return;
+    const int x = (int)(i % w), y = (int)(i / w);
+    const double xs = w > 1 ? -1.0 + (2.0 / (w - 1)) * x : -1.0;
+    const double ys = h > 1 ? -1.0 + (2.0 / (h - 1)) * y : -1.0;
+    const double xv = (w > 1 && x == w - 1) ? 1.0 : xs;  // numpy linspace ends exactly at 1
+    const double yv = (h > 1 && y == h - 1) ? 1.0 : ys;
+    double sn, cs;
+    sincos(k * (xv * xv + yv * yv), &sn, &cs);
+    const double a = (double)ap[i];
+    out[i] = make_float2((float)(a * cs), (float)(a * sn));
+}
+
+// out = |fftshift(F)|^2 (squares summed in double, then rounded to float)
+__global__ __launch_bounds__(256) void k_power_shift(const float2* __restrict__ F, int w, int h,
+                                                     float* __restrict__ out) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long)w * h) return;
+    const int x = (int)(i % w), y = (int)(i / w);
+    const int sx = (x + w / 2) % w, sy = (y + h / 2) % h;
+    const float2 v = F[(long)sy * w + sx];
+    out[i] = (float)((double)v.x * v.x + (double)v.y * v.y);
+}
+
+// ---------------------------------------------------------------------------
+// ghost: fractional Fourier transform (frft.py)
+
+// in: real image; out: complex fftshift(image)
+__global__ __launch_bounds__(256) void k_shift_real(const float* __restrict__ in, int w, int h,
+                                                    float2* __restrict__ out) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long)w * h) return;
+    const int x = (int)(i % w), y = (int)(i / w);
+    const int sx = (x + w / 2) % w, sy = (y + h / 2) % h;
+    out[i] = make_float2(in[(long)sy * w + sx], 0.0f);
+}
+
+__global__ __launch_bounds__(256) void k_flip(const float2* __restrict__ in, int w, int h,
+                                              float2* __restrict__ out) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long)w * h) return;
+    const int x = (int)(i % w), y = (int)(i / w);
+    out[i] = in[(long)(h - 1 - y) * w + (w - 1 - x)];
+}
+
+// chirp phase of frft.py:8-17 at (x, y): pi * (ky^2 / h + kx^2 / w) with
+// k the FFT-order frequency index
+__device__ __forceinline__ double chirp_phase(int x, int y, int w, int h) {
+    const double kx = x < (w + 1) / 2 ? x : x - w;
+    const double ky = y < (h + 1) / 2 ? y : y - h;
+    return 3.14159265358979323846 * (ky * ky / h + kx * kx / w);
+}
+
+// f2 = chirp1 * x, chirp2 in c2 (both complex64)
+__global__ __launch_bounds__(256) void k_frft_pre(const float2* __restrict__ x, int w, int h,
+                                                  double c1, double c2, float2* __restrict__ f2,
+                                                  float2* __restrict__ ch2) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long)w * h) return;
+    const double ph = chirp_phase((int)(i % w), (int)(i / w), w, h);
+    double s1, k1, s2, k2;
+    sincos(ph * c1, &s1, &k1);
+    sincos(ph * c2, &s2, &k2);
+    const double xr = x[i].x, xi = x[i].y;
+    f2[i] = make_float2((float)(k1 * xr - s1 * xi), (float)(k1 * xi + s1 * xr));
+    ch2[i] = make_float2((float)k2, (float)s2);
+}
+
+__global__ __launch_bounds__(256) void k_cmul_inplace(float2* __restrict__ a,
+                                                      const float2* __restrict__ b, long n) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) a[i] = cmul(a[i], b[i]);
+}
+
+// out = |fftshift(scale * chirp1 * y)| * amp
+__global__ __launch_bounds__(256) void k_frft_post(const float2* __restrict__ y, int w, int h,
+                                                   double c1, double sr, double si, double amp,
+                                                   float* __restrict__ out) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long)w * h) return;
+    const int x = (int)(i % w), yy = (int)(i / w);
+    const int sx = (x + w / 2) % w, sy = (yy + h / 2) % h;
+    const long j = (long)sy * w + sx;
+    double s1, k1;
+    sincos(chirp_phase(sx, sy, w, h) * c1, &s1, &k1);
+    const double ar = sr * k1 - si * s1, ai = sr * s1 + si * k1;  // scale * chirp1
+    const double vr = y[j].x, vi = y[j].y;
+    const double re = ar * vr - ai * vi, im = ar * vi + ai * vr;
+    out[i] = (float)sqrt(re * re + im * im) * (float)amp;
+}
+
+static inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace rf
+
+using namespace rf;
+
+extern "C" size_t rf_fft_workspace_size(int width, int height) {
+    return fft_workspace_bytes(width, height);
+}
+
+extern "C" int rf_fft2(void* data, int width, int height, int inverse, float scale, void* workspace,
+                       size_t workspace_bytes, rf_stream_t stream) {
+    if (!data) return set_error(RF_ERR_INVALID, "rf_fft2: NULL data");
+    if (!workspace || workspace_bytes < fft_workspace_bytes(width, height))
+        return set_error(RF_ERR_WORKSPACE, "rf_fft2: workspace too small");
+    FftPlan p;
+    int rc = fft_plan(width, height, workspace, (hipStream_t)stream, &p);
+    if (rc) return rc;
+    return fft_run(p, (float2*)data, inverse != 0, scale, (hipStream_t)stream);
+}
+
+extern "C" size_t rf_power_spectrum_workspace_size(int width, int height) {
+    return align_up((size_t)width * height * sizeof(float2), 256) + fft_workspace_bytes(width, height);
+}
+
+extern "C" int rf_power_spectrum(const float* aperture, int width, int height, float distance,
+                                 void* workspace, size_t workspace_bytes, float* spectrum,
+                                 rf_stream_t stream) {
+    if (!aperture || !spectrum) return set_error(RF_ERR_INVALID, "rf_power_spectrum: NULL");
+    if (!workspace || workspace_bytes < rf_power_spectrum_workspace_size(width, height))
+        return set_error(RF_ERR_WORKSPACE, "rf_power_spectrum: workspace too small");
+    hipStream_t s = (hipStream_t)stream;
+    float2* field = (float2*)workspace;
+    FftPlan p;
+    int rc = fft_plan(width, height,
+                      (char*)workspace + align_up((size_t)width * height * sizeof(float2), 256), s,
+                      &p);
+    if (rc) return rc;
+    // starburst.py:60-63: lambda = 560 nm in mm, d = max(1e-9, distance * 1e3) mm
+    const double wavelength = ((390.0 + 730.0) / 2.0) * 1e-6;
+    double d = (double)distance * 1e3;
+    if (d < 1e-9) d = 1e-9;
+    const double k = 3.14159265358979323846 / (wavelength * d);
+    const long n = (long)width * height;
+    hipLaunchKernelGGL(k_starburst_field, dim3(nblk(n)), dim3(256), 0, s, aperture, width, height, k,
+                       field);
+    rc = fft_run(p, field, false, 1.0f, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_power_shift, dim3(nblk(n)), dim3(256), 0, s, field, width, height, spectrum);
+    RF_LAUNCH_CHECK("rf_power_spectrum");
+    return RF_OK;
+}
+
+extern "C" size_t rf_ghost_workspace_size(int width, int height) {
+    return 3 * align_up((size_t)width * height * sizeof(float2), 256) +
+           fft_workspace_bytes(width, height);
+}
+
+extern "C" int rf_ghost(const float* aperture, int width, int height, float fstop,
+                        void* workspace, size_t workspace_bytes, float* ghost,
+                        rf_stream_t stream) {
+    if (!aperture || !ghost) return set_error(RF_ERR_INVALID, "rf_ghost: NULL");
+    if (!workspace || workspace_bytes < rf_ghost_workspace_size(width, height))
+        return set_error(RF_ERR_WORKSPACE, "rf_ghost: workspace too small");
+    hipStream_t s = (hipStream_t)stream;
+    const long n = (long)width * height;
+    const size_t plane = align_up((size_t)n * sizeof(float2), 256);
+    float2* a = (float2*)workspace;
+    float2* b = (float2*)((char*)workspace + plane);
+    float2* c = (float2*)((char*)workspace + 2 * plane);
+    FftPlan p;
+    int rc = fft_plan(width, height, (char*)workspace + 3 * plane, s, &p);
+    if (rc) return rc;
+    const float ortho = (float)(1.0 / sqrt((double)n));
+
+    // ghost.py:24: alpha = 0.15 * (lambda / 400) * (fstop / 18), lambda = 560
+    double alpha = 0.15 * (560.0 / 400.0) * ((double)fstop / 18.0);
+    hipLaunchKernelGGL(k_shift_real, dim3(nblk(n)), dim3(256), 0, s, aperture, width, height, a);
+    // frft.py:20-40: bring alpha into [0.5, 1.5)
+    double rel = fmod(alpha, 4.0);
+    if (rel < 0) rel += 4.0;
+    float2* x = a;
+    if (rel < 0.5) {
+        alpha = rel + 1;
+        rc = fft_run(p, a, true, ortho, s);
+    } else if (rel < 1.5) {
+        alpha = rel;
+    } else if (rel < 2.5) {
+        alpha = rel - 1;
+        rc = fft_run(p, a, false, ortho, s);
+    } else if (rel < 3.5) {
+        alpha = rel - 2;
+        hipLaunchKernelGGL(k_flip, dim3(nblk(n)), dim3(256), 0, s, a, width, height, b);
+        x = b;
+    } else {
+        alpha = rel - 3;
+        rc = fft_run(p, a, true, ortho, s);
+    }
+    if (rc) return rc;
+    // frft.py:43-62
+    const double phi = alpha * 3.14159265358979323846 / 2.0;
+    const double cot = 1.0 / tan(phi);
+    const double sq = sqrt(1.0 + cot * cot);
+    // scale = sqrt(1 - i cot) / sqrt(N): principal square root
+    const double mod = sqrt(sqrt(1.0 + cot * cot));
+    const double arg = 0.5 * atan2(-cot, 1.0);
+    const double sr = mod * cos(arg) / sqrt((double)n), si = mod * sin(arg) / sqrt((double)n);
+    float2* f2 = (x == a) ? b : a;
+    hipLaunchKernelGGL(k_frft_pre, dim3(nblk(n)), dim3(256), 0, s, x, width, height, cot - sq, sq, f2,
+                       c);
+    rc = fft_run(p, c, false, ortho, s);  // fft1 = fftn(chirp2, ortho)
+    if (rc) return rc;
+    rc = fft_run(p, f2, false, ortho, s);  // fft2 = fftn(chirp1 * x, ortho)
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_cmul_inplace, dim3(nblk(n)), dim3(256), 0, s, f2, c, n);
+    rc = fft_run(p, f2, true, ortho, s);  // ifftn(fft1 * fft2, ortho)
+    if (rc) return rc;
+    // ghost.py:27-29: fftshift, abs, float32, * sqrt(w h)
+    hipLaunchKernelGGL(k_frft_post, dim3(nblk(n)), dim3(256), 0, s, f2, width, height, cot - sq, sr,
+                       si, sqrt((double)width * height), ghost);
+    RF_LAUNCH_CHECK("rf_ghost");
+    return RF_OK;
+}

@@ -1,0 +1,253 @@
+// Device-wide scan and stable radix sort used by the tile binner.
+//
+// Both work on element counts that live in device memory (the binner only
+// learns how many (tile, quad-step) pairs a frame has on the device), so the
+// grids are sized for a capacity and blocks past the live count exit at once.
+// Nothing here synchronizes with the host.
+#include "primitives.h"
+
+namespace rf {
+
+static constexpr int SCAN_THREADS = 256;
+static constexpr int SCAN_ITEMS = 16;
+static constexpr int SCAN_CHUNK = SCAN_THREADS * SCAN_ITEMS;  // 4096
+
+__device__ __forceinline__ uint64_t wave_incl_scan_u64(uint64_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint64_t o = __shfl_up(v, d, 64);
+        if (lane >= d) v += o;
+    }
+    return v;
+}
+
+// Exclusive scan of one block's SCAN_THREADS values; returns the block total
+// through *total (all threads).
+__device__ __forceinline__ uint64_t block_excl_scan_u64(uint64_t v, uint64_t* total) {
+    __shared__ uint64_t s_wave[SCAN_THREADS / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint64_t incl = wave_incl_scan_u64(v);
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    uint64_t base = 0, sum = 0;
+#pragma unroll
+    for (int w = 0; w < SCAN_THREADS / 64; w++) {
+        uint64_t t = s_wave[w];
+        if (w < wave) base += t;
+        sum += t;
+    }
+    __syncthreads();
+    *total = sum;
+    return base + incl - v;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const uint32_t* __restrict__ in,
+                                                               const uint64_t* n_ptr, uint64_t n_fixed,
+                                                               uint64_t* __restrict__ block_sums) {
+    const uint64_t n = n_ptr ? *n_ptr : n_fixed;
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_CHUNK;
+    uint64_t s = 0;
+    if (base < n) {
+        for (int i = 0; i < SCAN_ITEMS; i++) {
+            uint64_t idx = base + (uint64_t)i * SCAN_THREADS + threadIdx.x;
+            if (idx < n) s += in[idx];
+        }
+    }
+    uint64_t total;
+    block_excl_scan_u64(s, &total);
+    if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+}
+
+// One block: exclusive scan of the block sums in place; writes the grand
+// total to *total_out.
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_sums(uint64_t* __restrict__ sums, int nblocks,
+                                                             uint64_t* __restrict__ total_out) {
+    uint64_t carry = 0;
+    for (int start = 0; start < nblocks; start += SCAN_THREADS) {
+        int i = start + threadIdx.x;
+        uint64_t v = i < nblocks ? sums[i] : 0;
+        uint64_t total;
+        uint64_t ex = block_excl_scan_u64(v, &total);
+        if (i < nblocks) sums[i] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0 && total_out) *total_out = carry;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const uint32_t* __restrict__ in,
+                                                              const uint64_t* n_ptr, uint64_t n_fixed,
+                                                              const uint64_t* __restrict__ block_offs,
+                                                              uint64_t* __restrict__ out) {
+    const uint64_t n = n_ptr ? *n_ptr : n_fixed;
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_CHUNK;
+    if (base >= n) return;
+    // each thread owns SCAN_ITEMS consecutive elements
+    const uint64_t first = base + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS];
+    uint64_t s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        uint64_t idx = first + i;
+        v[i] = idx < n ? in[idx] : 0u;
+        s += v[i];
+    }
+    uint64_t total;
+    uint64_t run = block_offs[blockIdx.x] + block_excl_scan_u64(s, &total);
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        uint64_t idx = first + i;
+        if (idx < n) out[idx] = run;
+        run += v[i];
+    }
+}
+
+size_t scan_workspace_bytes(uint64_t capacity) {
+    uint64_t nb = (capacity + SCAN_CHUNK - 1) / SCAN_CHUNK;
+    return align_up(nb * sizeof(uint64_t) + 64, 256);
+}
+
+int scan_exclusive_u32(const uint32_t* in, uint64_t* out, const uint64_t* n_dev, uint64_t capacity,
+                       uint64_t* total_dev, void* ws, hipStream_t s) {
+    int nb = (int)((capacity + SCAN_CHUNK - 1) / SCAN_CHUNK);
+    if (nb == 0) nb = 1;
+    uint64_t* sums = (uint64_t*)ws;
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_THREADS), 0, s, in, n_dev, capacity, sums);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_THREADS), 0, s, sums, nb, total_dev);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_THREADS), 0, s, in, n_dev, capacity, sums, out);
+    RF_LAUNCH_CHECK("scan_exclusive_u32");
+    return RF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// stable LSD radix sort of (uint32 key, uint32 value) pairs, 8-bit digits
+// ---------------------------------------------------------------------------
+
+static constexpr int RS_THREADS = 256;
+static constexpr int RS_ITEMS = 16;
+static constexpr int RS_CHUNK = RS_THREADS * RS_ITEMS;  // 4096
+
+__global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const uint32_t* __restrict__ keys,
+                                                            const uint64_t* __restrict__ n_ptr,
+                                                            int shift, int nblocks,
+                                                            uint32_t* __restrict__ hist) {
+    __shared__ uint32_t s_hist[256];
+    s_hist[threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t n = *n_ptr;
+    const uint64_t base = (uint64_t)blockIdx.x * RS_CHUNK;
+    if (base < n) {
+        for (int i = 0; i < RS_ITEMS; i++) {
+            uint64_t idx = base + (uint64_t)i * RS_THREADS + threadIdx.x;
+            if (idx < n) atomicAdd(&s_hist[(keys[idx] >> shift) & 255u], 1u);
+        }
+    }
+    __syncthreads();
+    // digit-major: all blocks of digit 0, then digit 1, ...
+    hist[(uint64_t)threadIdx.x * nblocks + blockIdx.x] = s_hist[threadIdx.x];
+}
+
+__global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
+    const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
+    uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
+    const uint64_t* __restrict__ n_ptr, int shift, int nblocks,
+    const uint64_t* __restrict__ digit_offs) {
+    __shared__ uint32_t s_run[256];
+    __shared__ uint32_t s_wcnt[RS_THREADS / 64][256];
+    const uint64_t n = *n_ptr;
+    const uint64_t base = (uint64_t)blockIdx.x * RS_CHUNK;
+    if (base >= n) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    s_run[threadIdx.x] = 0;
+#pragma unroll
+    for (int w = 0; w < RS_THREADS / 64; w++) s_wcnt[w][threadIdx.x] = 0;
+    __syncthreads();
+    for (int r = 0; r < RS_ITEMS; r++) {
+        const uint64_t idx = base + (uint64_t)r * RS_THREADS + threadIdx.x;
+        const bool valid = idx < n;
+        uint32_t key = valid ? keys_in[idx] : 0u;
+        uint32_t val = valid ? vals_in[idx] : 0u;
+        uint32_t d = (key >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            uint64_t m = __ballot(valid && ((d >> b) & 1u));
+            peers &= ((d >> b) & 1u) ? m : ~m;
+        }
+        const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
+        if (valid && rank == 0) s_wcnt[wave][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = s_run[d] + rank;
+            for (int w = 0; w < wave; w++) pos += s_wcnt[w][d];
+            uint64_t dst = digit_offs[(uint64_t)d * nblocks + blockIdx.x] + pos;
+            keys_out[dst] = key;
+            vals_out[dst] = val;
+        }
+        __syncthreads();
+        uint32_t add = 0;
+#pragma unroll
+        for (int w = 0; w < RS_THREADS / 64; w++) {
+            add += s_wcnt[w][threadIdx.x];
+            s_wcnt[w][threadIdx.x] = 0;
+        }
+        s_run[threadIdx.x] += add;
+        __syncthreads();
+    }
+}
+
+static int radix_blocks(uint64_t capacity) {
+    uint64_t nb = (capacity + RS_CHUNK - 1) / RS_CHUNK;
+    return nb == 0 ? 1 : (int)nb;
+}
+
+size_t radix_workspace_bytes(uint64_t capacity) {
+    uint64_t nb = (uint64_t)radix_blocks(capacity);
+    uint64_t nh = 256 * nb;
+    size_t b = 0;
+    b += align_up(2 * capacity * sizeof(uint32_t), 256);  // ping-pong keys+values
+    b += align_up(nh * sizeof(uint32_t), 256);           // histograms
+    b += align_up(nh * sizeof(uint64_t), 256);           // digit offsets
+    b += scan_workspace_bytes(nh);
+    return b;
+}
+
+int radix_sort_pairs(uint32_t* keys, uint32_t* vals, const uint64_t* n_dev, uint64_t capacity,
+                     int key_bits, void* ws, hipStream_t s) {
+    const int nb = radix_blocks(capacity);
+    const uint64_t nh = 256ull * nb;
+    char* p = (char*)ws;
+    uint32_t* k2 = (uint32_t*)p;
+    uint32_t* v2 = k2 + capacity;
+    p += align_up(2 * capacity * sizeof(uint32_t), 256);
+    uint32_t* hist = (uint32_t*)p;
+    p += align_up(nh * sizeof(uint32_t), 256);
+    uint64_t* offs = (uint64_t*)p;
+    p += align_up(nh * sizeof(uint64_t), 256);
+    void* scan_ws = p;
+    int passes = (key_bits + 7) / 8;
+    if (passes < 1) passes = 1;
+    // an even number of passes leaves the result in the caller's arrays
+    if (passes & 1) passes++;
+    uint32_t *ka = keys, *va = vals, *kb = k2, *vb = v2;
+    for (int pass = 0; pass < passes; pass++) {
+        int shift = 8 * pass;
+        if (shift > 24) shift = 24;  // extra pass for parity re-sorts the top digit (stable)
+        hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(RS_THREADS), 0, s, ka, n_dev, shift, nb, hist);
+        int rc = scan_exclusive_u32(hist, offs, nullptr, nh, nullptr, scan_ws, s);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(RS_THREADS), 0, s, ka, va, kb, vb, n_dev,
+                           shift, nb, offs);
+        RF_LAUNCH_CHECK("radix_sort_pairs");
+        uint32_t* t = ka;
+        ka = kb;
+        kb = t;
+        t = va;
+        va = vb;
+        vb = t;
+    }
+    return RF_OK;
+}
+
+}  // namespace rf

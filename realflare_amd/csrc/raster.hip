@@ -1,0 +1,767 @@
+// Binned quad rasterization of the traced ghost grids on MI355X.
+//
+// Reference: realflare/api/tasks/cl/rasterizing.cl (prim_shader 233-320,
+// vertex_shader 322-386, binner 388-469, rasterizer 471-663) driven by
+// RasterizingTask.rasterize (realflare/api/tasks/rasterizing.py:296-421).
+//
+// Pipeline of one frame (all on one stream, no host synchronization):
+//   k_prim     one thread per quad: energy-preserving intensity per
+//              wavelength, bounds over wavelengths, culling (as prim_shader).
+//   k_vertex   one thread per (ghost, ray, wavelength) vertex (vertex_shader).
+//   k_steps    per (wavelength pair, sub-wavelength) step: blend factor and
+//              CIE XYZ weight, accumulated in float exactly as the
+//              rasterizer's loop counters are.
+//   k_setup    one thread per (quad, step): the integer sample-space vertex
+//              positions, top-left edge biases, and the rectangle of 8x8
+//              pixel tiles the step can cover, cut to the reference's bins
+//              (bin_size) that the binner would have listed the quad in.
+//   scan+expand+stable radix sort: (tile, quad-step) pairs grouped by tile,
+//              each tile's pairs in ascending (quad, step) order, which is the
+//              order the reference's per-pixel loop visits them.
+//   k_raster   one wave per 8x8 tile: every lane owns one pixel, walks the
+//              tile's pairs, tests coverage (Hormann-Tarini, top-left rule,
+//              n-rooks samples), shades covered pixels and accumulates in
+//              registers; one coalesced write per pixel, y flipped.
+// A pixel therefore sums exactly the fragments the reference sums, in the
+// same order.
+#include "primitives.h"
+
+namespace rf {
+
+static constexpr int TILE = 8;
+
+struct Geom {
+    int P, W, g, R, Q, S, Wsub, maxw, AA, width, height, bin, ntx, nty, nbx, nby;
+    long N;       // P*W*R
+    long NV;      // P*R*W vertices
+    long NE;      // P*Q*S (quad, step) entries
+    float st, area_orig, min_area, intensity, ghost_scale;
+    int gw, gh, nspec;
+    int bin_exact;
+};
+
+struct Header {
+    uint64_t required;  // (tile, quad-step) pairs the frame needs
+    uint64_t n_sort;    // pairs binned (0 on overflow)
+    uint32_t overflow;  // required > capacity: nothing was rastered
+    uint32_t pad_;
+};
+
+struct alignas(16) Setup {
+    int vx[4], vy[4];
+    int flags;  // bits 0..4: top-left of edges 01,12,23,30,20
+    uint32_t r0, r1;  // tile rect (tx0 | ty0 << 16), (tx1 | ty1 << 16)
+    int pad_;
+};
+
+__constant__ unsigned char c_sub_offsets[15] = {0, 1, 0, 1, 2, 0, 3, 4, 1, 6, 2, 5, 0, 3, 7};
+
+__device__ __forceinline__ long floordiv(long a, long b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
+__device__ __forceinline__ long ceildiv(long a, long b) { return -floordiv(-a, b); }
+
+__device__ __forceinline__ void quad_vertexes(int g, int quad, int q[4]) {
+    int row = quad / (g - 1), col = quad % (g - 1);
+    q[0] = row * g + col;
+    q[1] = q[0] + 1;
+    q[2] = q[1] + g;
+    q[3] = q[0] + g;
+}
+
+__device__ __forceinline__ float edge_f(float ax, float ay, float bx, float by, float cx, float cy) {
+#pragma clang fp contract(off)
+    return (ax - bx) * (cy - ay) - (ay - by) * (cx - ax);
+}
+
+__device__ __forceinline__ int edge_i(int ax, int ay, int bx, int by, int cx, int cy) {
+    uint32_t t0 = ((uint32_t)ax - (uint32_t)bx) * ((uint32_t)cy - (uint32_t)ay);
+    uint32_t t1 = ((uint32_t)ay - (uint32_t)by) * ((uint32_t)cx - (uint32_t)ax);
+    return (int)(t0 - t1);
+}
+
+__device__ __forceinline__ bool is_top_left(int ax, int ay, int bx, int by) {
+    int ex = wsub(bx, ax), ey = wsub(by, ay);
+    return (ey == 0 && ex > 0) || ey > 0;
+}
+
+// ---------------------------------------------------------------------------
+// prim_shader (rasterizing.cl:233-320).  Intensities the source leaves
+// unwritten (wavelengths after the first culled one) are written as 0, the
+// value a freshly cleared buffer holds.
+__global__ __launch_bounds__(256) void k_prim(const float* __restrict__ planes, Geom G,
+                                              float4* __restrict__ bounds,
+                                              float* __restrict__ intens) {
+#pragma clang fp contract(off)
+    const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (long)G.P * G.Q) return;
+    const int path = (int)(idx / G.Q), quad = (int)(idx % G.Q);
+    int qv[4];
+    quad_vertexes(G.g, quad, qv);
+    const float* px_ = planes;
+    const float* py_ = planes + G.N;
+    const float* rr_ = planes + 4 * G.N;
+    const float* rf_ = planes + 5 * G.N;
+    float b0 = INFINITY, b1 = INFINITY, b2 = -INFINITY, b3 = -INFINITY;
+    int invalid_rrel = 0;
+    for (int w = 0; w < G.W; w++) {
+        float* out_i = intens + idx * G.W + w;
+        if (isnan(b0)) {
+            *out_i = 0.0f;
+            continue;
+        }
+        const long ro = ((long)path * G.W + w) * G.R;
+        float rx[4], ry[4], rrel[4], refl[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            rx[i] = px_[ro + qv[i]];
+            ry[i] = py_[ro + qv[i]];
+            rrel[i] = rr_[ro + qv[i]];
+            refl[i] = rf_[ro + qv[i]];
+        }
+        float cx[4] = {0, 0, 0, 0}, cy[4] = {0, 0, 0, 0};
+        int valid = 0;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            if (!isnan(refl[i])) {
+                // compaction into the first `valid` slots
+                if (valid == 0) { cx[0] = rx[i]; cy[0] = ry[i]; }
+                else if (valid == 1) { cx[1] = rx[i]; cy[1] = ry[i]; }
+                else if (valid == 2) { cx[2] = rx[i]; cy[2] = ry[i]; }
+                else { cx[3] = rx[i]; cy[3] = ry[i]; }
+                valid++;
+            }
+        }
+        float area;
+        if (valid == 4) {
+            float a0 = edge_f(cx[0], cy[0], cx[1], cy[1], cx[2], cy[2]);
+            float a1 = edge_f(cx[0], cy[0], cx[2], cy[2], cx[3], cy[3]);
+            area = fabsf(a0 + a1) / 2;
+        } else if (valid == 3) {
+            area = fabsf(edge_f(cx[0], cy[0], cx[1], cy[1], cx[2], cy[2]));
+            cx[3] = cx[2];
+            cy[3] = cy[2];
+            rrel[3] = rrel[2];
+        } else {
+            b0 = NAN;
+            *out_i = 0.0f;
+            continue;
+        }
+        if (rrel[0] > 1 && rrel[1] > 1 && rrel[2] > 1 && rrel[3] > 1) invalid_rrel++;
+        float area_actual = fmaxf(area, G.min_area);
+        *out_i = area_actual > 0 ? G.area_orig / area_actual : 0.0f;
+        b0 = fminf(b0, fminf(fminf(cx[0], cx[1]), fminf(cx[2], cx[3])));
+        b1 = fminf(b1, fminf(fminf(cy[0], cy[1]), fminf(cy[2], cy[3])));
+        b2 = fmaxf(b2, fmaxf(fmaxf(cx[0], cx[1]), fmaxf(cx[2], cx[3])));
+        b3 = fmaxf(b3, fmaxf(fmaxf(cy[0], cy[1]), fmaxf(cy[2], cy[3])));
+    }
+    if (invalid_rrel == G.W) b0 = NAN;
+    bounds[idx] = make_float4(b0, b1, b2, b3);
+}
+
+// vertex_shader (rasterizing.cl:322-386); Vertex = 32 bytes
+// (pos.xy, uv.xy, rrel, reflectance, intensity, pad).
+__global__ __launch_bounds__(256) void k_vertex(const float* __restrict__ planes,
+                                                const float* __restrict__ intens, Geom G,
+                                                float4* __restrict__ vertexes) {
+#pragma clang fp contract(off)
+    const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= G.NV) return;
+    const int w = (int)(idx % G.W);
+    const int ray = (int)((idx / G.W) % G.R);
+    const int path = (int)(idx / ((long)G.W * G.R));
+    const long ri = ((long)path * G.W + w) * G.R + ray;
+    const float refl = planes[5 * G.N + ri];
+    float4 a = make_float4(0, 0, 0, 0), b = make_float4(0, refl, 0, 0);
+    if (!isnan(refl)) {
+        const int x = ray % G.g, y = ray / G.g;
+        int nb[4];
+        nb[0] = (y - 1) * (G.g - 1) + (x - 1);
+        nb[1] = nb[0] + 1;
+        nb[2] = y * (G.g - 1) + (x - 1);
+        nb[3] = nb[2] + 1;
+        float it = 0.0f;
+        int count = 0;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            if (nb[i] < 0 || nb[i] >= G.Q) continue;
+            float v = intens[((long)path * G.Q + nb[i]) * G.W + w];
+            if (v > 0) {
+                it += v;
+                ++count;
+            }
+        }
+        it /= (float)(count > 1 ? count : 1);
+        a.x = planes[ri] * G.st + (float)G.width / 2;
+        a.y = planes[G.N + ri] * G.st + (float)G.height / 2;
+        a.z = planes[2 * G.N + ri];
+        a.w = planes[3 * G.N + ri];
+        b.x = planes[4 * G.N + ri];
+        b.z = it;
+    }
+    vertexes[2 * idx] = a;
+    vertexes[2 * idx + 1] = b;
+}
+
+// Step tables: blend factor and XYZ weight per (wavelength pair, sub step),
+// accumulated in float like the rasterizer's loop variables
+// (rasterizing.cl:556-557, 616-617, 644, 650-651).
+__global__ void k_steps(const float* __restrict__ spectrum, Geom G, float4* __restrict__ table) {
+#pragma clang fp contract(off)
+    const int i = threadIdx.x + blockIdx.x * blockDim.x;
+    if (i >= G.maxw) return;
+    const float sub_step = 1.0f / (float)G.Wsub;
+    const float w_step = sub_step / (float)G.W;
+    float sub_pos = 0.0f;
+    float w_pos = ((float)i + 0.5f) / (float)G.W;
+    for (int j = 0; j < G.Wsub; j++) {
+        float x = 1.0f, y = 1.0f, z = 1.0f;
+        if (G.W > 1) {
+            x = sample_bilinear(spectrum + 0, G.nspec, 1, 4, w_pos, 0.0f, true);
+            y = sample_bilinear(spectrum + 1, G.nspec, 1, 4, w_pos, 0.0f, true);
+            z = sample_bilinear(spectrum + 2, G.nspec, 1, 4, w_pos, 0.0f, true);
+        }
+        table[i * G.Wsub + j] = make_float4(sub_pos, x, y, z);
+        sub_pos += sub_step;
+        w_pos += w_step;
+    }
+}
+
+__device__ __forceinline__ void load_pos(const float4* __restrict__ vtx, long vi, long NV, float& x,
+                                         float& y) {
+    if (vi < NV) {
+        float4 a = vtx[2 * vi];
+        x = a.x;
+        y = a.y;
+    } else {
+        x = 0.0f;
+        y = 0.0f;
+    }
+}
+
+// Per (quad, step): integer vertices, edge biases, tile rectangle, pair count.
+__global__ __launch_bounds__(256) void k_setup(const float4* __restrict__ vtx,
+                                               const float4* __restrict__ bounds,
+                                               const float4* __restrict__ table, Geom G,
+                                               Setup* __restrict__ setup,
+                                               uint32_t* __restrict__ counts) {
+#pragma clang fp contract(off)
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= G.NE) return;
+    const long prim = e / G.S;
+    const int step = (int)(e % G.S);
+    const int wl = step / G.Wsub;
+    uint32_t count = 0;
+    const float4 b = bounds[prim];
+    Setup rec;
+    rec.flags = 0;
+    rec.r0 = rec.r1 = 0;
+    rec.pad_ = 0;
+    for (int j = 0; j < 4; j++) rec.vx[j] = rec.vy[j] = 0;
+    if (!isnan(b.x)) {
+        const int path = (int)(prim / G.Q), quad = (int)(prim % G.Q);
+        int qv[4];
+        quad_vertexes(G.g, quad, qv);
+        const float blend = table[step].x;
+        int vx[4], vy[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const long base = ((long)path * G.R + qv[j]) * G.W + wl;
+            float ax, ay, bx, by;
+            load_pos(vtx, base, G.NV, ax, ay);
+            load_pos(vtx, base + 1, G.NV, bx, by);
+            float mx = ax + (bx - ax) * blend;
+            float my = ay + (by - ay) * blend;
+            vx[j] = rf_to_int(mx * (float)G.AA);
+            vy[j] = rf_to_int(my * (float)G.AA);
+            rec.vx[j] = vx[j];
+            rec.vy[j] = vy[j];
+        }
+        const bool collinear = edge_i(vx[0], vy[0], vx[1], vy[1], vx[2], vy[2]) == 0 &&
+                               edge_i(vx[0], vy[0], vx[2], vy[2], vx[3], vy[3]) == 0;
+        if (!collinear) {
+            int flags = 0;
+            flags |= is_top_left(vx[0], vy[0], vx[1], vy[1]) ? 1 : 0;
+            flags |= is_top_left(vx[1], vy[1], vx[2], vy[2]) ? 2 : 0;
+            flags |= is_top_left(vx[2], vy[2], vx[3], vy[3]) ? 4 : 0;
+            flags |= is_top_left(vx[3], vy[3], vx[0], vy[0]) ? 8 : 0;
+            flags |= is_top_left(vx[2], vy[2], vx[0], vy[0]) ? 16 : 0;
+            rec.flags = flags;
+            // covered samples lie inside the integer vertices' bounding box
+            long sx0 = min(min(vx[0], vx[1]), min(vx[2], vx[3]));
+            long sx1 = max(max(vx[0], vx[1]), max(vx[2], vx[3]));
+            long sy0 = min(min(vy[0], vy[1]), min(vy[2], vy[3]));
+            long sy1 = max(max(vy[0], vy[1]), max(vy[2], vy[3]));
+            long x0 = floordiv(sx0, G.AA), x1 = floordiv(sx1, G.AA);
+            long y0 = floordiv(sy0, G.AA), y1 = floordiv(sy1, G.AA);
+            // bins the binner lists this quad in (rasterizing.cl:449-453)
+            const int sbx0 = wadd(rf_to_int(b.x * G.st), G.width / 2);
+            const int sby0 = wadd(rf_to_int(b.y * G.st), G.height / 2);
+            const int sbx1 = wadd(rf_to_int(b.z * G.st), G.width / 2);
+            const int sby1 = wadd(rf_to_int(b.w * G.st), G.height / 2);
+            long bx0 = ceildiv((long)sbx0 - G.bin, G.bin), bx1 = floordiv(sbx1, G.bin);
+            long by0 = ceildiv((long)sby0 - G.bin, G.bin), by1 = floordiv(sby1, G.bin);
+            bx0 = max(bx0, 0L);
+            by0 = max(by0, 0L);
+            bx1 = min(bx1, (long)G.nbx - 1);
+            by1 = min(by1, (long)G.nby - 1);
+            x0 = max(x0, max(bx0 * G.bin, 0L));
+            y0 = max(y0, max(by0 * G.bin, 0L));
+            x1 = min(x1, min((bx1 + 1) * G.bin - 1, (long)G.width - 1));
+            y1 = min(y1, min((by1 + 1) * G.bin - 1, (long)G.height - 1));
+            if (x0 <= x1 && y0 <= y1) {
+                const uint32_t tx0 = (uint32_t)(x0 / TILE), tx1 = (uint32_t)(x1 / TILE);
+                const uint32_t ty0 = (uint32_t)(y0 / TILE), ty1 = (uint32_t)(y1 / TILE);
+                rec.r0 = tx0 | (ty0 << 16);
+                rec.r1 = tx1 | (ty1 << 16);
+                count = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
+            }
+        }
+    }
+    setup[e] = rec;
+    counts[e] = count;
+}
+
+__global__ void k_finalize(Header* __restrict__ h, uint64_t capacity) {
+    const uint64_t need = h->required;
+    h->overflow = need > capacity ? 1u : 0u;
+    h->n_sort = need > capacity ? 0 : need;
+}
+
+__global__ __launch_bounds__(256) void k_expand(const Setup* __restrict__ setup,
+                                                const uint32_t* __restrict__ counts,
+                                                const uint64_t* __restrict__ offsets,
+                                                const Header* __restrict__ h, Geom G,
+                                                uint32_t* __restrict__ keys,
+                                                uint32_t* __restrict__ vals) {
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= G.NE || h->overflow) return;
+    const uint32_t count = counts[e];
+    if (!count) return;
+    const Setup& rec = setup[e];
+    const uint32_t tx0 = rec.r0 & 0xffff, ty0 = rec.r0 >> 16;
+    const uint32_t tx1 = rec.r1 & 0xffff;
+    const uint32_t rw = tx1 - tx0 + 1;
+    const uint64_t off = offsets[e];
+    for (uint32_t k = 0; k < count; k++) {
+        const uint32_t tx = tx0 + k % rw, ty = ty0 + k / rw;
+        keys[off + k] = ty * (uint32_t)G.ntx + tx;
+        vals[off + k] = (uint32_t)e;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ p, long n, uint32_t v) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(256) void k_ranges(const uint32_t* __restrict__ keys,
+                                                const Header* __restrict__ h,
+                                                uint32_t* __restrict__ tstart,
+                                                uint32_t* __restrict__ tend) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const uint64_t n = h->n_sort;
+    if (i >= n) return;
+    const uint32_t k = keys[i];
+    if (i == 0 || keys[i - 1] != k) tstart[k] = (uint32_t)i;
+    if (i == n - 1 || keys[i + 1] != k) tend[k] = (uint32_t)(i + 1);
+}
+
+// rasterizing.cl:55-93 with the edge biases precomputed per quad step.
+__device__ __forceinline__ bool intersect_quad(int px, int py, const int* x, const int* y, int flags) {
+    const int b01 = (flags & 1) ? 0 : -1, b12 = (flags & 2) ? 0 : -1, b23 = (flags & 4) ? 0 : -1;
+    const int b30 = (flags & 8) ? 0 : -1, b20 = (flags & 16) ? 0 : -1;
+    const bool w01 = wadd(edge_i(x[0], y[0], x[1], y[1], px, py), b01) >= 0;
+    const bool w12 = wadd(edge_i(x[1], y[1], x[2], y[2], px, py), b12) >= 0;
+    const bool w23 = wadd(edge_i(x[2], y[2], x[3], y[3], px, py), b23) >= 0;
+    const bool w30 = wadd(edge_i(x[3], y[3], x[0], y[0], px, py), b30) >= 0;
+    const bool w20 = wadd(edge_i(x[2], y[2], x[0], y[0], px, py), b20) >= 0;
+    const bool front = (w20 && w01 && w12 && (w23 || w30)) || (!w20 && w23 && w30 && (w01 || w12));
+    const bool back = (!w20 && !w01 && !w12 && (!w23 || !w30)) || (w20 && !w23 && !w30 && (!w01 || !w12));
+    return front || back;
+}
+
+// rasterizing.cl:95-148 (mean value coordinates of a quad)
+__device__ __forceinline__ void barycentric_quad(int px, int py, const int* x, const int* y,
+                                                 float* w) {
+#pragma clang fp contract(off)
+    float sx[4], sy[4], a[4], d[4], r[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        sx[i] = (float)wsub(x[i], px);
+        sy[i] = (float)wsub(y[i], py);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int j = (i + 1) & 3;
+        a[i] = sx[i] * sy[j] - sx[j] * sy[i];
+        d[i] = sx[i] * sx[j] + sy[i] * sy[j];
+        r[i] = sqrtf(sx[i] * sx[i] + sy[i] * sy[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        if (r[i] == 0) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) w[k] = (k == i) ? 1.0f : 0.0f;
+            return;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+        if (a[i] == 0) a[i] = 1e-3f;
+    float t[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) t[i] = (r[i] * r[(i + 1) & 3] - d[i]) / a[i];
+    float ww[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) ww[i] = (t[(i + 3) & 3] + t[i]) / r[i];
+    const float sum = ww[0] + ww[1] + ww[2] + ww[3];
+#pragma unroll
+    for (int i = 0; i < 4; i++) w[i] = ww[i] / sum;
+}
+
+__global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ vtx,
+                                               const float4* __restrict__ bounds,
+                                               const float* __restrict__ ghost,
+                                               const float4* __restrict__ table,
+                                               const Setup* __restrict__ setup,
+                                               const uint32_t* __restrict__ vals,
+                                               const uint32_t* __restrict__ tstart,
+                                               const uint32_t* __restrict__ tend,
+                                               const Header* __restrict__ h, Geom G,
+                                               float4* __restrict__ image) {
+#pragma clang fp contract(off)
+    if (h->overflow) return;
+    const int tile = blockIdx.x;
+    const int tx = tile % G.ntx, ty = tile / G.ntx;
+    const int lane = threadIdx.x;
+    const int x = tx * TILE + (lane & (TILE - 1)), y = ty * TILE + (lane / TILE);
+    const bool inside = x < G.width && y < G.height;
+    const int px = x * G.AA, py = y * G.AA;
+    const int pcx = px + G.AA / 2, pcy = py + G.AA / 2;
+    const int bxp = x / G.bin, byp = y / G.bin;
+    float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f;
+    const uint32_t start = tstart[tile], end = tend[tile];
+    for (uint32_t k = start; k < end; k++) {
+        const uint32_t e = vals[k];
+        const Setup rec = setup[e];
+        const long prim = e / G.S;
+        int hits = 0;
+        bool ok = inside;
+        if (ok && !G.bin_exact) {
+            const float4 b = bounds[prim];
+            const int sbx0 = wadd(rf_to_int(b.x * G.st), G.width / 2);
+            const int sby0 = wadd(rf_to_int(b.y * G.st), G.height / 2);
+            const int sbx1 = wadd(rf_to_int(b.z * G.st), G.width / 2);
+            const int sby1 = wadd(rf_to_int(b.w * G.st), G.height / 2);
+            const int bx0 = bxp * G.bin, by0 = byp * G.bin;
+            ok = sbx0 <= bx0 + G.bin && sbx1 >= bx0 && sby0 <= by0 + G.bin && sby1 >= by0;
+        }
+        if (ok) {
+            for (int s = 0; s < G.AA; s++) {
+                const int sx = px + s, sy = py + c_sub_offsets[G.AA - 1 + s];
+                if (intersect_quad(sx, sy, rec.vx, rec.vy, rec.flags)) hits++;
+            }
+        }
+        if (!__any(hits > 0)) continue;
+        const int step = (int)(e % G.S);
+        const int wl = step / G.Wsub;
+        const float4 tab = table[step];
+        const float blend = tab.x;
+        const int path = (int)(prim / G.Q), quad = (int)(prim % G.Q);
+        int qv[4];
+        quad_vertexes(G.g, quad, qv);
+        float u[4], v[4], rr[4], rf[4], it[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const long base = ((long)path * G.R + qv[j]) * G.W + wl;
+            const float4 a0 = vtx[2 * base], a1 = vtx[2 * base + 1];
+            float4 b0 = make_float4(0, 0, 0, 0), b1 = make_float4(0, 0, 0, 0);
+            if (base + 1 < G.NV) {
+                b0 = vtx[2 * (base + 1)];
+                b1 = vtx[2 * (base + 1) + 1];
+            }
+            u[j] = a0.z + (b0.z - a0.z) * blend;
+            v[j] = a0.w + (b0.w - a0.w) * blend;
+            rr[j] = a1.x + (b1.x - a1.x) * blend;
+            rf[j] = a1.y + (b1.y - a1.y) * blend;
+            it[j] = a1.z + (b1.z - a1.z) * blend;
+        }
+        if (hits > 0) {
+            float w[4];
+            barycentric_quad(pcx, pcy, rec.vx, rec.vy, w);
+            // fragment_shader, rasterizing.cl:471-506
+            float uu = w[0] * u[0] + w[1] * u[1] + w[2] * u[2] + w[3] * u[3];
+            float vv = w[0] * v[0] + w[1] * v[1] + w[2] * v[2] + w[3] * v[3];
+            uu = (uu / G.ghost_scale + 1.0f) / 2.0f;
+            vv = (vv / G.ghost_scale + 1.0f) / 2.0f;
+            const float gi = sample_bilinear(ghost, G.gw, G.gh, 1, uu, vv, false);
+            const float rel = w[0] * rr[0] + w[1] * rr[1] + w[2] * rr[2] + w[3] * rr[3];
+            const float ri = smoothstepf(1.0f, 0.95f, rel);
+            float ai = w[0] * it[0] + w[1] * it[1] + w[2] * it[2] + w[3] * it[3];
+            float ci = w[0] * rf[0] + w[1] * rf[1] + w[2] * rf[2] + w[3] * rf[3];
+            ai = fmaxf(ai, 0.0f);
+            ci = fmaxf(ci, 0.0f);
+            const float frag = 0.0f + gi * ri * ai * ci * (float)hits;
+            r0 += tab.y * frag;
+            r1 += tab.z * frag;
+            r2 += tab.w * frag;
+        }
+    }
+    if (!inside) return;
+    float4 out = make_float4(0, 0, 0, 0);
+    if (r0 > 0 || r1 > 0 || r2 > 0) {
+        const float f = G.intensity / (float)(G.W * G.AA * G.Wsub);
+        float o[4];
+        xyz_to_ap1(r0 * f, r1 * f, r2 * f, o);
+        out = make_float4(o[0], o[1], o[2], o[3]);
+    }
+    image[(long)(G.height - 1 - y) * G.width + x] = out;
+}
+
+// ---------------------------------------------------------------------------
+
+static int make_geom(const rf_raster_params* p, Geom* G) {
+    if (!p) return set_error(RF_ERR_INVALID, "raster: params is NULL");
+    if (p->path_count < 0 || p->wavelength_count < 1 || p->grid_count < 2)
+        return set_error(RF_ERR_INVALID, "raster: need wavelength_count >= 1 and grid_count >= 2");
+    if (p->width < 1 || p->height < 1 || p->width > 65536 * TILE || p->height > 65536 * TILE)
+        return set_error(RF_ERR_INVALID, "raster: resolution %dx%d out of range", p->width, p->height);
+    if (!(p->anti_aliasing == 1 || p->anti_aliasing == 2 || p->anti_aliasing == 4 ||
+          p->anti_aliasing == 8))
+        return set_error(RF_ERR_INVALID, "raster: anti_aliasing must be 1, 2, 4 or 8 (got %d)",
+                         p->anti_aliasing);
+    if (p->wavelength_sub_count < 1)
+        return set_error(RF_ERR_INVALID, "raster: wavelength_sub_count must be >= 1");
+    if (p->bin_size < 1) return set_error(RF_ERR_INVALID, "raster: bin_size must be >= 1");
+    if (p->ghost_width < 1 || p->ghost_height < 1)
+        return set_error(RF_ERR_INVALID, "raster: ghost texture size must be positive");
+    G->P = p->path_count;
+    G->W = p->wavelength_count;
+    G->g = p->grid_count;
+    G->R = G->g * G->g;
+    G->Q = (G->g - 1) * (G->g - 1);
+    G->Wsub = p->wavelength_sub_count;
+    G->maxw = G->W - 1 > 1 ? G->W - 1 : 1;
+    G->S = G->maxw * G->Wsub;
+    G->AA = p->anti_aliasing;
+    G->width = p->width;
+    G->height = p->height;
+    G->bin = p->bin_size;
+    G->ntx = (G->width + TILE - 1) / TILE;
+    G->nty = (G->height + TILE - 1) / TILE;
+    G->nbx = (G->width + G->bin - 1) / G->bin;
+    G->nby = (G->height + G->bin - 1) / G->bin;
+    G->N = (long)G->P * G->W * G->R;
+    G->NV = G->N;
+    G->NE = (long)G->P * G->Q * G->S;
+    G->st = p->screen_transform;
+    G->area_orig = p->area_orig;
+    G->min_area = p->min_area;
+    G->intensity = p->intensity;
+    G->ghost_scale = p->ghost_scale;
+    G->gw = p->ghost_width;
+    G->gh = p->ghost_height;
+    G->nspec = p->spectrum_count;
+    G->bin_exact = (G->bin % TILE) == 0;
+    if (G->W > 1 && G->nspec < 1)
+        return set_error(RF_ERR_INVALID, "raster: spectrum_count must be >= 1");
+    if (G->NE > 0xffffffffL)
+        return set_error(RF_ERR_INVALID, "raster: %ld quad steps exceed 2^32", G->NE);
+    return RF_OK;
+}
+
+static inline unsigned blocks_for(long n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+
+struct Layout {
+    size_t header, table, setup, counts, offsets, scan_ws, keys, vals, radix_ws, tstart, tend, total;
+};
+
+static Layout layout(const Geom& G, uint64_t cap) {
+    Layout L;
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        size_t at = o;
+        o += align_up(bytes, 256);
+        return at;
+    };
+    L.header = take(sizeof(Header));
+    L.table = take((size_t)G.S * sizeof(float4));
+    L.setup = take((size_t)G.NE * sizeof(Setup));
+    L.counts = take((size_t)G.NE * sizeof(uint32_t));
+    L.offsets = take((size_t)G.NE * sizeof(uint64_t));
+    L.scan_ws = take(scan_workspace_bytes((uint64_t)G.NE));
+    L.keys = take(cap * sizeof(uint32_t));
+    L.vals = take(cap * sizeof(uint32_t));
+    L.radix_ws = take(radix_workspace_bytes(cap));
+    const size_t ntiles = (size_t)G.ntx * G.nty;
+    L.tstart = take(ntiles * sizeof(uint32_t));
+    L.tend = take(ntiles * sizeof(uint32_t));
+    L.total = o;
+    return L;
+}
+
+static int raster_impl(const float4* vtx, const float4* bounds, const float* ghost,
+                       const float* spectrum, const Geom& G, uint64_t cap, void* ws, size_t ws_bytes,
+                       float4* image, hipStream_t s) {
+    const Layout L = layout(G, cap);
+    if (!ws || ws_bytes < L.total)
+        return set_error(RF_ERR_WORKSPACE, "rf_rasterize: workspace %zu bytes < %zu needed", ws_bytes,
+                         L.total);
+    char* w = (char*)ws;
+    Header* h = (Header*)(w + L.header);
+    float4* table = (float4*)(w + L.table);
+    Setup* setup = (Setup*)(w + L.setup);
+    uint32_t* counts = (uint32_t*)(w + L.counts);
+    uint64_t* offsets = (uint64_t*)(w + L.offsets);
+    uint32_t* keys = (uint32_t*)(w + L.keys);
+    uint32_t* vals = (uint32_t*)(w + L.vals);
+    uint32_t* tstart = (uint32_t*)(w + L.tstart);
+    uint32_t* tend = (uint32_t*)(w + L.tend);
+    const long ntiles = (long)G.ntx * G.nty;
+
+    hipLaunchKernelGGL(k_steps, dim3(blocks_for(G.maxw, 64)), dim3(64), 0, s, spectrum, G, table);
+    if (G.NE > 0) {
+        hipLaunchKernelGGL(k_setup, dim3(blocks_for(G.NE, 256)), dim3(256), 0, s, vtx, bounds, table,
+                           G, setup, counts);
+    }
+    int rc = scan_exclusive_u32(counts, offsets, nullptr, (uint64_t)G.NE, &h->required,
+                                w + L.scan_ws, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1), 0, s, h, cap);
+    if (G.NE > 0) {
+        hipLaunchKernelGGL(k_expand, dim3(blocks_for(G.NE, 256)), dim3(256), 0, s, setup, counts,
+                           offsets, h, G, keys, vals);
+    }
+    int bits = 1;
+    while ((1L << bits) < ntiles) bits++;
+    rc = radix_sort_pairs(keys, vals, &h->n_sort, cap, bits, w + L.radix_ws, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(ntiles, 256)), dim3(256), 0, s, tstart, ntiles, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(ntiles, 256)), dim3(256), 0, s, tend, ntiles, 0u);
+    if (cap > 0) {
+        hipLaunchKernelGGL(k_ranges, dim3(blocks_for((long)cap, 256)), dim3(256), 0, s, keys, h,
+                           tstart, tend);
+    }
+    hipLaunchKernelGGL(k_raster, dim3((unsigned)ntiles), dim3(64), 0, s, vtx, bounds, ghost, table,
+                       setup, vals, tstart, tend, h, G, image);
+    RF_LAUNCH_CHECK("rf_rasterize");
+    return RF_OK;
+}
+
+struct FlareLayout {
+    size_t bounds, intens, vertexes, raster, total;
+};
+
+static FlareLayout flare_layout(const Geom& G, uint64_t cap) {
+    FlareLayout F;
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        size_t at = o;
+        o += align_up(bytes, 256);
+        return at;
+    };
+    F.raster = take(layout(G, cap).total);  // first: its header sits at offset 0
+    F.bounds = take((size_t)G.P * G.Q * sizeof(float4));
+    F.intens = take((size_t)G.P * G.Q * G.W * sizeof(float));
+    F.vertexes = take((size_t)G.NV * 2 * sizeof(float4));
+    F.total = o;
+    return F;
+}
+
+}  // namespace rf
+
+using namespace rf;
+
+extern "C" int rf_prim_shader(const float* planes, const rf_raster_params* prm, float* bounds,
+                              float* intensities, rf_stream_t stream) {
+    Geom G;
+    int rc = make_geom(prm, &G);
+    if (rc) return rc;
+    if ((long)G.P * G.Q == 0) return RF_OK;
+    if (!planes || !bounds || !intensities) return set_error(RF_ERR_INVALID, "rf_prim_shader: NULL");
+    hipLaunchKernelGGL(k_prim, dim3(blocks_for((long)G.P * G.Q, 256)), dim3(256), 0,
+                       (hipStream_t)stream, planes, G, (float4*)bounds, intensities);
+    RF_LAUNCH_CHECK("rf_prim_shader");
+    return RF_OK;
+}
+
+extern "C" int rf_vertex_shader(const float* planes, const float* intensities,
+                                const rf_raster_params* prm, void* vertexes, rf_stream_t stream) {
+    Geom G;
+    int rc = make_geom(prm, &G);
+    if (rc) return rc;
+    if (G.NV == 0) return RF_OK;
+    if (!planes || !intensities || !vertexes)
+        return set_error(RF_ERR_INVALID, "rf_vertex_shader: NULL");
+    hipLaunchKernelGGL(k_vertex, dim3(blocks_for(G.NV, 256)), dim3(256), 0, (hipStream_t)stream,
+                       planes, intensities, G, (float4*)vertexes);
+    RF_LAUNCH_CHECK("rf_vertex_shader");
+    return RF_OK;
+}
+
+extern "C" size_t rf_rasterize_workspace_size(const rf_raster_params* prm, uint64_t pair_capacity) {
+    Geom G;
+    if (make_geom(prm, &G)) return 0;
+    return layout(G, pair_capacity).total;
+}
+
+extern "C" int rf_rasterize(const void* vertexes, const float* bounds, const float* ghost,
+                            const float* light_spectrum, const rf_raster_params* prm,
+                            uint64_t pair_capacity, void* workspace, size_t workspace_bytes,
+                            float* image, rf_stream_t stream) {
+    Geom G;
+    int rc = make_geom(prm, &G);
+    if (rc) return rc;
+    if (!image || !ghost || (G.W > 1 && !light_spectrum) || (G.NE > 0 && (!vertexes || !bounds)))
+        return set_error(RF_ERR_INVALID, "rf_rasterize: NULL buffer");
+    return raster_impl((const float4*)vertexes, (const float4*)bounds, ghost, light_spectrum, G,
+                       pair_capacity, workspace, workspace_bytes, (float4*)image,
+                       (hipStream_t)stream);
+}
+
+extern "C" size_t rf_flare_workspace_size(const rf_raster_params* prm, uint64_t pair_capacity) {
+    Geom G;
+    if (make_geom(prm, &G)) return 0;
+    return flare_layout(G, pair_capacity).total;
+}
+
+extern "C" int rf_flare(const float* planes, const float* ghost, const float* light_spectrum,
+                        const rf_raster_params* prm, uint64_t pair_capacity, void* workspace,
+                        size_t workspace_bytes, float* image, rf_stream_t stream) {
+    Geom G;
+    int rc = make_geom(prm, &G);
+    if (rc) return rc;
+    const FlareLayout F = flare_layout(G, pair_capacity);
+    if (!workspace || workspace_bytes < F.total)
+        return set_error(RF_ERR_WORKSPACE, "rf_flare: workspace %zu bytes < %zu needed",
+                         workspace_bytes, F.total);
+    if (!image || !ghost || (G.W > 1 && !light_spectrum) || (G.NV > 0 && !planes))
+        return set_error(RF_ERR_INVALID, "rf_flare: NULL buffer");
+    char* w = (char*)workspace;
+    float4* bounds = (float4*)(w + F.bounds);
+    float* intens = (float*)(w + F.intens);
+    float4* vtx = (float4*)(w + F.vertexes);
+    hipStream_t s = (hipStream_t)stream;
+    if ((long)G.P * G.Q > 0) {
+        hipLaunchKernelGGL(k_prim, dim3(blocks_for((long)G.P * G.Q, 256)), dim3(256), 0, s, planes, G,
+                           bounds, intens);
+        hipLaunchKernelGGL(k_vertex, dim3(blocks_for(G.NV, 256)), dim3(256), 0, s, planes, intens, G,
+                           vtx);
+    }
+    return raster_impl(vtx, bounds, ghost, light_spectrum, G, pair_capacity, w + F.raster,
+                       F.bounds - F.raster, (float4*)image, s);
+}
+
+extern "C" int rf_raster_status(const void* workspace, uint64_t* required_pairs, int* overflow,
+                                rf_stream_t stream) {
+    if (!workspace || !required_pairs || !overflow)
+        return set_error(RF_ERR_INVALID, "rf_raster_status: NULL");
+    Header h;
+    hipError_t e = hipMemcpyAsync(&h, workspace, sizeof(Header), hipMemcpyDeviceToHost,
+                                  (hipStream_t)stream);
+    if (e != hipSuccess) return check_hip(e, "rf_raster_status");
+    e = hipStreamSynchronize((hipStream_t)stream);
+    if (e != hipSuccess) return check_hip(e, "rf_raster_status");
+    *required_pairs = h.required;
+    *overflow = (int)h.overflow;
+    return RF_OK;
+}

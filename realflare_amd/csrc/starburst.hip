@@ -1,0 +1,128 @@
+// Starburst (diffraction) image on MI355X.
+//
+// Reference: the `starburst` kernel (realflare/api/tasks/cl/starburst.cl:1-87)
+// with geometry.cl, noise.cl and color.cl, launched by
+// StarburstTask.starburst (realflare/api/tasks/starburst.py:83-153).  One
+// thread per output pixel takes `samples` jittered, wavelength-scaled
+// bilinear lookups of the power spectrum, weights them by the CIE XYZ table
+// and converts to ACEScg (AP1).  The XYZ weight of every sample depends only
+// on the sample index, so it is evaluated once per workgroup into LDS.
+#include <cfloat>
+#include <cmath>
+
+#include "rf_common.h"
+
+namespace rf {
+
+static constexpr int SB_MAX_CACHED = 1024;
+
+__device__ __forceinline__ float sb_noise(float x, float y, float z) {
+#pragma clang fp contract(off)
+    float v = sinf(x * 112.9898f + y * 179.233f + z * 237.212f) * 43758.5453f;
+    return fminf(v - floorf(v), 0x1.fffffep-1f);
+}
+
+__global__ __launch_bounds__(256) void k_starburst(const float* __restrict__ spectrum, int sw, int sh,
+                                                   const float* __restrict__ light, int light_n,
+                                                   rf_starburst_params p, float4* __restrict__ out) {
+#pragma clang fp contract(off)
+    __shared__ float4 s_xyz[SB_MAX_CACHED];
+    const float pi = 3.14159265358979323846f;
+    const int samples = p.samples;
+    const bool cached = samples <= SB_MAX_CACHED;
+    if (cached) {
+        for (int t = threadIdx.x; t < samples; t += blockDim.x) {
+            const float step = (float)t / (float)samples;
+            float4 v;
+            v.x = sample_bilinear(light + 0, light_n, 1, 4, step, 0.0f, false);
+            v.y = sample_bilinear(light + 1, light_n, 1, 4, step, 0.0f, false);
+            v.z = sample_bilinear(light + 2, light_n, 1, 4, step, 0.0f, false);
+            v.w = sample_bilinear(light + 3, light_n, 1, 4, step, 0.0f, false);
+            s_xyz[t] = v;
+        }
+        __syncthreads();
+    }
+    const int w = p.width, h = p.height;
+    const int px = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int py = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (px >= w || py >= h) return;
+    float nx = 2.0f * ((float)px / (float)(w - 1)) - 1.0f;
+    float ny = 2.0f * ((float)py / (float)(h - 1)) - 1.0f;
+    nx = nx - p.offset[0];
+    ny = ny - p.offset[1];
+    nx = nx / fmaxf(p.size[0], FLT_MIN);
+    ny = ny / fmaxf(p.size[1], FLT_MIN);
+    const int lambda_delta = 730 - 390;
+    float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f, r3 = 0.0f;
+    const float fx = (float)px, fy = (float)py;
+    for (int t = 0; t < samples; ++t) {
+        const float step = (float)t / (float)samples;
+        const float wavelength = step * (float)lambda_delta + 390.0f;
+        const float seed = (float)(t * 4);
+        float x = nx, y = ny;
+        const float radius = p.blur * sqrtf(sb_noise(fx, fy, seed));
+        const float alpha = sb_noise(fx, fy, seed + 1) * 2 * pi;
+        x += cosf(alpha) * radius;
+        y += sinf(alpha) * radius;
+        const float sc = fmaxf(1 - (1 - (wavelength / 560.0f)) * 1.0f, FLT_MIN);
+        x = x / sc;
+        y = y / sc;
+        const float sign = (sb_noise(fx, fy, seed + 2) > 0.5f) ? 1.0f : -1.0f;
+        const float angle =
+            sign * p.rotation * (1.0f - powf(sb_noise(fx, fy, seed + 3), p.rotation_weight));
+        const float sa = sinf(angle), ca = cosf(angle);
+        const float rx = x * ca + y * sa, ry = y * ca - x * sa;
+        const float u = rx * 0.5f + 0.5f, v = ry * 0.5f + 0.5f;
+        const float fi = sample_bilinear(spectrum, sw, sh, 1, u, v, false);
+        float4 xyz;
+        if (cached) {
+            xyz = s_xyz[t];
+        } else {
+            xyz.x = sample_bilinear(light + 0, light_n, 1, 4, step, 0.0f, false);
+            xyz.y = sample_bilinear(light + 1, light_n, 1, 4, step, 0.0f, false);
+            xyz.z = sample_bilinear(light + 2, light_n, 1, 4, step, 0.0f, false);
+            xyz.w = sample_bilinear(light + 3, light_n, 1, 4, step, 0.0f, false);
+        }
+        r0 += xyz.x * fi;
+        r1 += xyz.y * fi;
+        r2 += xyz.z * fi;
+        r3 += xyz.w * fi;
+    }
+    r0 /= (float)samples;
+    r1 /= (float)samples;
+    r2 /= (float)samples;
+    r3 /= (float)samples;
+    if (!isnan(p.vignetting[0])) {
+        const float sdf = sqrtf(nx * nx + ny * ny) - 0.0f;
+        const float f = smoothstepf(p.vignetting[1], p.vignetting[0], sdf);
+        r0 *= f;
+        r1 *= f;
+        r2 *= f;
+        r3 *= f;
+    }
+    r0 *= p.intensity;
+    r1 *= p.intensity;
+    r2 *= p.intensity;
+    float o[4];
+    xyz_to_ap1(r0, r1, r2, o);
+    out[(long)py * w + px] = make_float4(o[0], o[1], o[2], o[3]);
+}
+
+}  // namespace rf
+
+extern "C" int rf_starburst(const float* spectrum, int spectrum_width, int spectrum_height,
+                            const float* light_spectrum, int spectrum_count,
+                            const rf_starburst_params* prm, float* image, rf_stream_t stream) {
+    using namespace rf;
+    if (!prm || !spectrum || !light_spectrum || !image)
+        return set_error(RF_ERR_INVALID, "rf_starburst: NULL");
+    if (prm->width < 1 || prm->height < 1 || prm->samples < 1 || spectrum_width < 1 ||
+        spectrum_height < 1 || spectrum_count < 1)
+        return set_error(RF_ERR_INVALID, "rf_starburst: empty size or samples < 1");
+    dim3 grid((prm->width + 15) / 16, (prm->height + 15) / 16);
+    hipLaunchKernelGGL(k_starburst, grid, dim3(256), 0, (hipStream_t)stream, spectrum,
+                       spectrum_width, spectrum_height, light_spectrum, spectrum_count, *prm,
+                       (float4*)image);
+    RF_LAUNCH_CHECK("rf_starburst");
+    return RF_OK;
+}
