@@ -1,0 +1,241 @@
+"""Benchmark: rays traced per second through a full 2048x2048 FLARE_STARBURST
+frame on one MI355X.
+
+Workload (the reference's drop-in full-size configuration): a seeded
+synthetic 12-surface double-Gauss-like lens (f/2 stop mid-stack), all 66
+two-reflection ghosts, 128x128 rays per ghost, 8 wavelengths over 400-700
+nm, 2048x2048 RGBA fp32 output, 9-blade starburst aperture at 1024x1024
+with 8 spectral samples; the remaining render settings are the reference
+defaults.  A step is one whole frame: ghost aperture + FrFT, trace,
+prim/vertex shaders, binner, rasterizer, starburst aperture, power
+spectrum, starburst, composition.  All inputs are resident on the device.
+
+value = rays per frame / seconds per frame (rays = ghosts x wavelengths x
+grid^2).  Per-stage times use the reference's stage names.  The roofline
+object describes the dominant kernel, timed with HIP events on the stream
+it runs on.  cpu_baseline times the oracle (the C restatement of the
+reference kernels, OpenMP over all host cores) on a bounded sample of the
+same frame and extrapolates to a frame.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+# Algorithmic work per unit (frozen table; see the per-event counts in the
+# design notes): FLOPs per traced ray of the full workload, and bytes the
+# rasterizer must move at minimum per frame.
+TRACE_FLOP_PER_EVENT_REFRACT = 89
+TRACE_FLOP_PER_EVENT_REFLECT = 155
+TRACE_FLOP_INIT = 76
+TRACE_FLOP_STOP = 11
+TRACE_FLOP_SENSOR = 14
+TRACE_FLOP_SELLMEIER = 15
+
+
+def trace_flops(pipe) -> float:
+    """Algorithmic FLOPs of one trace launch: per ray, init + every surface
+    event of its ghost path (counted from raytracing.cl, each +,-,*,/,
+    sqrt and transcendental = 1), Sellmeier per glass side."""
+    L = pipe.lens_count
+    stop = pipe.lens_model.aperture_index
+    ior = [e.refractive_index for e in pipe.lens_model.lens_elements] + [1.0]
+    total = 0.0
+    for b1, b2 in pipe.paths:
+        seq = list(range(0, b1 + 1)) + list(range(b1 - 1, b2 - 1, -1)) + list(range(b2 + 1, L))
+        f = TRACE_FLOP_INIT
+        for k, el in enumerate(seq):
+            if el == stop:
+                f += TRACE_FLOP_STOP
+            elif el == L - 1:
+                f += TRACE_FLOP_SENSOR
+            else:
+                reflect = (el == b1 and k == b1) or (el == b2 and k == 2 * b1 - b2)
+                f += TRACE_FLOP_PER_EVENT_REFLECT if reflect else TRACE_FLOP_PER_EVENT_REFRACT
+                glass = (ior[el] > 1) + (el > 0 and ior[el - 1] > 1)
+                f += TRACE_FLOP_SELLMEIER * glass
+        total += f
+    return total * pipe.W * pipe.R
+
+
+def raster_bytes(pipe) -> float:
+    """Minimum bytes the rasterizer moves per frame: each vertex's 28 useful
+    bytes read once, the RGBA fp32 image written once."""
+    w, h = pipe.resolution
+    return pipe.P * pipe.R * pipe.W * 28.0 + w * h * 16.0
+
+
+def cpu_baseline(pipe, inp_name: str = 'full') -> dict:
+    """Oracle (C restatement of the reference kernels, OpenMP) on a bounded
+    sample of the frame, extrapolated to one frame."""
+    sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+    sys.path.insert(0, os.path.join(ROOT, 'tests'))
+    import numpy as np
+
+    import oracle
+    from helpers import trace_inputs
+    from raster_helpers import params_for
+
+    from realflare_amd.api.color import light_spectrum
+
+    oracle.build()
+    inp = trace_inputs(inp_name)
+    g = inp['grid_count']
+    t0 = time.perf_counter()
+    rays = oracle.trace(inp['lens'], inp['paths'], inp['wavelengths'], inp['aperture_index'],
+                        inp['coating_min_ior'], g, inp['grid_length'], inp['direction'])
+    t_trace = time.perf_counter() - t0
+    ghost = pipe.ghost_tex.cpu().numpy()
+    prm = params_for(inp, pipe.resolution, ghost.shape)
+    t0 = time.perf_counter()
+    bounds, intens = oracle.prim_shader(rays, g, prm.area_orig, prm.min_area)
+    vtx = oracle.vertex_shader(rays, intens, g, prm.screen_transform, pipe.resolution)
+    t_pv = time.perf_counter() - t0
+    w, h = pipe.resolution
+    rows = [int((i + 0.5) * h / 6) for i in range(6)]
+    t0 = time.perf_counter()
+    for y in rows:
+        oracle.rasterize(vtx, bounds, prm.wavelength_sub_count, g, prm.anti_aliasing,
+                         prm.intensity, prm.ghost_scale, ghost, light_spectrum(),
+                         prm.screen_transform, prm.bin_size, pipe.resolution, rows=(y, y + 1))
+    t_rows = time.perf_counter() - t0
+    t_raster = t_rows / len(rows) * h
+    # starburst: power spectrum (numpy, as the reference) + kernel on 4 rows
+    ap = pipe.star_ap.cpu().numpy()
+    t0 = time.perf_counter()
+    spec = oracle.power_spectrum(ap, pipe.project.flare.starburst.distance)
+    t_fft = time.perf_counter() - t0
+    sp = pipe.star_prm
+    t0 = time.perf_counter()
+    oracle.starburst((w, 4), spec, light_spectrum(), sp.samples, sp.blur, sp.rotation,
+                     sp.rotation_weight, tuple(sp.vignetting), sp.intensity, tuple(sp.offset),
+                     tuple(sp.size))
+    t_star = (time.perf_counter() - t0) / 4 * h
+    frame_s = t_trace + t_pv + t_raster + t_fft + t_star
+    return {
+        'value': pipe.rays / frame_s,
+        'unit': 'rays/s',
+        'cores': oracle.threads(),
+        'kind': 'port',
+        'sample': (f'oracle trace of all {pipe.rays} rays ({t_trace:.2f} s), prim+vertex shaders '
+                   f'({t_pv:.2f} s), rasterizer on {len(rows)} full-width rows spread over the '
+                   f'frame ({t_rows:.2f} s, x{h / len(rows):.0f} to a frame), numpy power '
+                   f'spectrum ({t_fft:.2f} s), starburst on 4 rows (x{h / 4:.0f}); '
+                   f'est. {frame_s:.1f} s per frame'),
+        'frame_s_estimate': frame_s,
+        'stages_s': {'RaytracingTask.run': t_trace, 'prim+vertex_shader': t_pv,
+                     'rasterizer': t_raster, 'fourier_spectrum': t_fft, 'starburst': t_star},
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=10)
+    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--workload', default='full', choices=['full', 'small'])
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f'bench.py measures one GPU; --gpus {args.gpus} is not supported (the flare path '
+              'has no multi-GPU sharding in this build)', file=sys.stderr)
+        return 2
+
+    import torch
+
+    from realflare_amd.pipeline import FramePipeline
+    from realflare_amd.synthetic import workload
+
+    assert torch.cuda.is_available(), 'bench.py needs a HIP device'
+    wl = workload(args.workload)
+    pipe = FramePipeline(wl.project, wl.lens_model, wl.paths, wl.wavelengths)
+    print(f'# workload {args.workload}: {pipe.P} ghosts x {pipe.W} wavelengths x {pipe.g}^2 rays '
+          f'= {pipe.rays} rays, {pipe.resolution[0]}x{pipe.resolution[1]}', file=sys.stderr,
+          flush=True)
+    pairs = pipe.fit_capacity()
+    print(f'# binning pairs per frame: {pairs}', file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        pipe.render()
+    torch.cuda.synchronize()
+
+    # timed region: exactly `steps` frames
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start.record()
+    for _ in range(args.steps):
+        pipe.render()
+    stop.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = start.elapsed_time(stop) / args.steps
+    _, overflow = pipe.raster_status()
+    assert not overflow
+
+    # per-stage times (separate pass, events around every stage)
+    events: dict = {}
+    for _ in range(args.steps):
+        pipe.render_timed(events)
+    torch.cuda.synchronize()
+    stages = {k: sum(a.elapsed_time(b) for a, b in v) / len(v) for k, v in events.items()}
+    print('# stage ms: ' + ', '.join(f'{k} {v:.3f}' for k, v in stages.items()), file=sys.stderr,
+          flush=True)
+
+    dominant = max(stages, key=stages.get)
+    if dominant == 'raytrace':
+        flops = trace_flops(pipe)
+        achieved = flops / (stages['raytrace'] * 1e-3) / 1e12
+        roofline = {'kernel': 'k_trace (rf_trace_planes)', 'bound': 'valu',
+                    'achieved': achieved, 'peak': FP32_PEAK_TFLOPS, 'unit': 'TFLOP/s',
+                    'frac': achieved / FP32_PEAK_TFLOPS, 'traffic': None}
+    else:
+        nbytes = raster_bytes(pipe)
+        achieved = nbytes / (stages[dominant] * 1e-3) / 1e9
+        roofline = {'kernel': 'k_raster (rf_raster_draw)' if dominant == 'rasterizer' else dominant,
+                    'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                    'frac': achieved / HBM_PEAK_GBS, 'traffic': None}
+    trace_tflops = trace_flops(pipe) / (stages['raytrace'] * 1e-3) / 1e12
+
+    line = {
+        'metric': 'rays traced/s & ms per 2048^2 flare frame, batched vs plain; adjoint ms/frame',
+        'value': pipe.rays / (ms * 1e-3),
+        'unit': 'rays/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': ms,
+        'frames_per_s': 1000.0 / ms,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f32',
+        'data': 'synthetic (seeded 12-surface lens, reference default render settings)',
+        'config': {'workload': f'{args.workload}: FLARE_STARBURST frame, {pipe.P} ghosts x '
+                               f'{pipe.W} wavelengths x {pipe.g}^2 rays, '
+                               f'{pipe.resolution[0]}x{pipe.resolution[1]}, starburst '
+                               f'{pipe.sres[0]}^2 x {pipe.star_prm.samples} samples',
+                   'rays_per_frame': pipe.rays, 'binning_pairs': pairs},
+        'wall_s': wall,
+        'stages_ms': stages,
+        'trace_rays_per_s': pipe.rays / (stages['raytrace'] * 1e-3),
+        'trace_tflops': trace_tflops,
+        'roofline': roofline,
+    }
+    if not args.no_cpu_baseline:
+        print('# cpu baseline (oracle on a bounded sample)...', file=sys.stderr, flush=True)
+        line['cpu_baseline'] = cpu_baseline(pipe, args.workload)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == '__main__':
+    sys.exit(main())

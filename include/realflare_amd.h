@@ -145,6 +145,19 @@ int rf_rasterize(const void *vertexes, const float *bounds, const float *ghost,
                  rf_stream_t stream);
 
 /*
+ * rf_rasterize in its two halves, on the same workspace: rf_raster_bin is
+ * the `binner` (rasterizing.cl:388-469, host py:366-385) and rf_raster_draw
+ * the `rasterizer` (rasterizing.cl:524-663, host py:387-418).  rf_raster_draw
+ * reads what the last rf_raster_bin on that workspace left there.
+ */
+int rf_raster_bin(const void *vertexes, const float *bounds, const float *light_spectrum,
+                  const rf_raster_params *prm, uint64_t pair_capacity, void *workspace,
+                  size_t workspace_bytes, rf_stream_t stream);
+int rf_raster_draw(const void *vertexes, const float *bounds, const float *ghost,
+                   const rf_raster_params *prm, uint64_t pair_capacity, void *workspace,
+                   size_t workspace_bytes, float *image, rf_stream_t stream);
+
+/*
  * The whole RasterizingTask.run (rasterizing.py:423-439): prim_shader,
  * vertex_shader, binner and rasterizer from trace planes to image, with all
  * intermediates in the workspace (rf_flare_workspace_size).
@@ -235,6 +248,14 @@ typedef struct rf_starburst_params {
 int rf_starburst(const float *spectrum, int spectrum_width, int spectrum_height,
                  const float *light_spectrum, int spectrum_count,
                  const rf_starburst_params *prm, float *image, rf_stream_t stream);
+
+/*
+ * FLARE_STARBURST composition, out = flare + (flare + starburst) over RGBA
+ * pixels, the reference's `array = flare.copy(); array += flare + starburst`
+ * (engine.py:174-187), which counts the flare twice.
+ */
+int rf_compose_flare_starburst(const float *flare, const float *starburst, long pixel_count,
+                               float *out, rf_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -340,3 +340,59 @@ def to_reference_rays(rays: np.ndarray) -> np.ndarray:
     for name in ('pos', 'dir', 'pos_apt', 'rrel', 'reflectance'):
         out[name] = rays[name]
     return out
+
+
+def _scalar(name, precision, restype, argtypes):
+    f = _fn(name, precision)
+    f.restype = restype
+    f.argtypes = argtypes
+    return f
+
+
+def fresnel(theta0, n1, n2, precision='f64'):
+    """raytracing.cl:1-16."""
+    d = ctypes.c_double
+    return _scalar('rfo_fresnel', precision, d, [d, d, d])(theta0, n1, n2)
+
+
+def fresnel_ar(theta0, lam, d1, n0, n1, n2, precision='f64'):
+    """raytracing.cl:18-59."""
+    d = ctypes.c_double
+    return _scalar('rfo_fresnel_ar', precision, d, [d] * 6)(theta0, lam, d1, n0, n1, n2)
+
+
+def dispersion(lam, coefficients, precision='f64'):
+    """raytracing.cl:107-120."""
+    c = np.zeros(8, np.float32)
+    c[:6] = coefficients
+    f = _scalar('rfo_dispersion', precision, ctypes.c_double, [ctypes.c_double, ctypes.c_void_p])
+    return f(lam, c.ctypes.data)
+
+
+def intersect_quad(px, py, xs, ys, precision='f32') -> bool:
+    """rasterizing.cl:55-93."""
+    x = np.asarray(xs, np.int32)
+    y = np.asarray(ys, np.int32)
+    f = _scalar('rfo_intersect_quad', precision, ctypes.c_int,
+                [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p])
+    return bool(f(px, py, x.ctypes.data, y.ctypes.data))
+
+
+def barycentric_quad(px, py, xs, ys, precision='f32') -> np.ndarray:
+    """rasterizing.cl:95-148."""
+    x = np.asarray(xs, np.int32)
+    y = np.asarray(ys, np.int32)
+    w = np.zeros(4, np.float64)
+    f = _scalar('rfo_barycentric_quad', precision, None,
+                [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p])
+    f(px, py, x.ctypes.data, y.ctypes.data, w.ctypes.data)
+    return w
+
+
+def sample_bilinear(tex, s, t, edge=False, precision='f32') -> float:
+    """OpenCL 1.2 bilinear filter (normalized coordinates), one channel."""
+    tex = np.ascontiguousarray(tex, np.float32)
+    f = _scalar('rfo_sample_bilinear', precision, ctypes.c_double,
+                [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                 ctypes.c_int])
+    return f(tex.ctypes.data, tex.shape[1], tex.shape[0], s, t, int(edge))

@@ -889,6 +889,30 @@ int FN(rfo_starburst)(int w, int h, const float *spectrum_img, int sw, int sh, c
     return 0;
 }
 
+/* Single-function entry points for the unit tests of the oracle itself. */
+double FN(rfo_fresnel)(double theta0, double n1, double n2) {
+    return (double)FN(fresnel)((REAL)theta0, (REAL)n1, (REAL)n2);
+}
+double FN(rfo_fresnel_ar)(double theta0, double lambda, double d1, double n0, double n1,
+                          double n2) {
+    return (double)FN(fresnel_ar)((REAL)theta0, (REAL)lambda, (REAL)d1, (REAL)n0, (REAL)n1,
+                                  (REAL)n2);
+}
+double FN(rfo_dispersion)(double lambda, const float *coefficients) {
+    return (double)FN(dispersion)((REAL)lambda, coefficients);
+}
+int FN(rfo_intersect_quad)(int px, int py, const int *x, const int *y) {
+    return FN(intersect_quad)(px, py, x, y);
+}
+void FN(rfo_barycentric_quad)(int px, int py, const int *x, const int *y, double *w) {
+    REAL ww[4];
+    FN(barycentric_quad)(px, py, x, y, ww);
+    for (int i = 0; i < 4; i++) w[i] = (double)ww[i];
+}
+double FN(rfo_sample_bilinear)(const float *tex, int w, int h, double s, double t, int edge) {
+    return (double)FN(sample_bilinear)(tex, w, h, 1, (REAL)s, (REAL)t, edge);
+}
+
 #undef V3
 #undef V2
 #undef RAY

@@ -1,0 +1,146 @@
+"""Render engine.
+
+Mirrors ``Engine`` (``realflare/api/engine.py:31-291``): one instance of
+every task, a renderer per ``RenderElement``, ``render(project) -> bool``
+with the reference's error handling (invalid input and device errors are
+logged and ``render`` still returns True; other exceptions return False),
+and the FLARE_STARBURST composition, which in the reference adds the flare
+twice (``array = flare.copy(); array += flare + starburst``,
+engine.py:174-187).  Rendered images are handed to ``image_rendered``
+callbacks instead of a Qt signal.
+"""
+from __future__ import annotations
+
+import logging
+from collections import OrderedDict
+from typing import Callable
+
+import numpy as np
+import torch
+
+from .. import native
+from ..utils.timing import timer
+from .data import Project, RealflareError, RenderElement
+from .tasks.aperture import GhostApertureTask, StarburstApertureTask
+from .tasks.device import Image
+from .tasks.ghost import GhostTask
+from .tasks.preprocessing import PreprocessTask
+from .tasks.rasterizing import RasterizingTask
+from .tasks.raytracing import IntersectionsTask, RaytracingTask
+from .tasks.starburst import StarburstTask
+
+logger = logging.getLogger(__name__)
+
+
+class Engine:
+    def __init__(self) -> None:
+        self._elements: list[RenderElement] = []
+        self._emit_cache: dict = {}
+        self.image_rendered: list[Callable] = []
+        self.initialized = False
+
+    def _init(self) -> None:
+        native.lib()  # fails loudly when the HIP library is not built
+        self.renderers: dict[RenderElement, Callable] = OrderedDict()
+        self.renderers[RenderElement.STARBURST_APERTURE] = self.starburst_aperture
+        self.renderers[RenderElement.GHOST_APERTURE] = self.ghost_aperture
+        self.renderers[RenderElement.STARBURST] = self.starburst
+        self.renderers[RenderElement.GHOST] = self.ghost
+        self.renderers[RenderElement.FLARE] = self.flare
+        self.renderers[RenderElement.FLARE_STARBURST] = self.flare_starburst
+        self.renderers[RenderElement.DIAGRAM] = self.diagram
+        self.ghost_aperture_task = GhostApertureTask()
+        self.starburst_aperture_task = StarburstApertureTask()
+        self.ghost_task = GhostTask()
+        self.starburst_task = StarburstTask()
+        self.raytracing_task = RaytracingTask()
+        self.intersection_task = IntersectionsTask()
+        self.rasterizing_task = RasterizingTask()
+        self.preprocess_task = PreprocessTask(self.raytracing_task)
+        self.initialized = True
+
+    def elements(self) -> list[RenderElement]:
+        return self._elements
+
+    def set_elements(self, elements: list[RenderElement]) -> None:
+        self._elements = list(elements)
+        self._emit_cache = {}
+
+    def starburst_aperture(self, project: Project) -> Image:
+        return self.starburst_aperture_task.run(project)
+
+    def starburst(self, project: Project) -> Image:
+        aperture = self.starburst_aperture(project)
+        return self.starburst_task.run(project, aperture)
+
+    def ghost_aperture(self, project: Project) -> Image:
+        return self.ghost_aperture_task.run(project)
+
+    def ghost(self, project: Project) -> Image:
+        aperture = self.ghost_aperture(project)
+        return self.ghost_task.run(project, aperture)
+
+    def flare(self, project: Project) -> Image:
+        if project.render.debug_ghost_enabled:
+            path_indexes = (project.render.debug_ghost,)
+        else:
+            path_indexes = self.preprocess_task.run(project)
+        if project.flare.light.image_file_enabled:
+            raise RealflareError('image-based lights are not supported by this build')
+        ghost = self.ghost(project)
+        rays = self.raytracing_task.run(project, path_indexes)
+        return self.rasterizing_task.run(project, rays, ghost)
+
+    def flare_starburst(self, project: Project) -> Image:
+        flare = self.flare(project)
+        array = flare.tensor.clone()
+        starburst = self.starburst(project)
+        array += flare.tensor + starburst.tensor
+        return Image(tensor=array, args=(flare.args, starburst.args))
+
+    def diagram(self, project: Project):
+        path_indexes = (project.diagram.debug_ghost,)
+        return self.intersection_task.run(project, path_indexes)
+
+    @timer
+    def render(self, project: Project) -> bool:
+        if not self.initialized:
+            self._init()
+        try:
+            for element in self._elements:
+                renderer = self.renderers.get(element)
+                if renderer:
+                    image = renderer(project)
+                    if torch.cuda.is_available():
+                        torch.cuda.synchronize()
+                    self.emit_image(image, element)
+        except RealflareError as e:
+            logger.error(e)
+        except (native.NativeError, torch.cuda.OutOfMemoryError) as e:
+            logger.exception(e)
+            logger.error(
+                'Render failed. This is most likely because the GPU ran out of memory. '
+                'Consider lowering the settings.'
+            )
+        except Exception as e:  # noqa: BLE001 (the reference's catch-all)
+            logger.exception(e)
+            return False
+        return True
+
+    def emit_image(self, image, element: RenderElement) -> None:
+        _hash = hash(image)
+        if self._emit_cache.get(element) != _hash:
+            self._emit_cache[element] = _hash
+            for callback in self.image_rendered:
+                callback(image, element)
+
+
+def render_array(project: Project, element: RenderElement = RenderElement.FLARE) -> np.ndarray:
+    """Convenience: render one element and return it as a host array."""
+    engine = Engine()
+    engine.set_elements([element])
+    out = {}
+    engine.image_rendered.append(lambda image, el: out.__setitem__(el, image))
+    if not engine.render(project) or element not in out:
+        raise RealflareError(f'render of {element.name} failed')
+    return out[element].array

@@ -599,52 +599,86 @@ static Layout layout(const Geom& G, uint64_t cap) {
     return L;
 }
 
-static int raster_impl(const float4* vtx, const float4* bounds, const float* ghost,
-                       const float* spectrum, const Geom& G, uint64_t cap, void* ws, size_t ws_bytes,
-                       float4* image, hipStream_t s) {
+struct RasterPtrs {
+    Header* h;
+    float4* table;
+    Setup* setup;
+    uint32_t *counts, *keys, *vals, *tstart, *tend;
+    uint64_t* offsets;
+    void *scan_ws, *radix_ws;
+};
+
+static int raster_ptrs(const Geom& G, uint64_t cap, void* ws, size_t ws_bytes, RasterPtrs* r,
+                       const char* who) {
     const Layout L = layout(G, cap);
     if (!ws || ws_bytes < L.total)
-        return set_error(RF_ERR_WORKSPACE, "rf_rasterize: workspace %zu bytes < %zu needed", ws_bytes,
+        return set_error(RF_ERR_WORKSPACE, "%s: workspace %zu bytes < %zu needed", who, ws_bytes,
                          L.total);
     char* w = (char*)ws;
-    Header* h = (Header*)(w + L.header);
-    float4* table = (float4*)(w + L.table);
-    Setup* setup = (Setup*)(w + L.setup);
-    uint32_t* counts = (uint32_t*)(w + L.counts);
-    uint64_t* offsets = (uint64_t*)(w + L.offsets);
-    uint32_t* keys = (uint32_t*)(w + L.keys);
-    uint32_t* vals = (uint32_t*)(w + L.vals);
-    uint32_t* tstart = (uint32_t*)(w + L.tstart);
-    uint32_t* tend = (uint32_t*)(w + L.tend);
-    const long ntiles = (long)G.ntx * G.nty;
+    r->h = (Header*)(w + L.header);
+    r->table = (float4*)(w + L.table);
+    r->setup = (Setup*)(w + L.setup);
+    r->counts = (uint32_t*)(w + L.counts);
+    r->offsets = (uint64_t*)(w + L.offsets);
+    r->scan_ws = w + L.scan_ws;
+    r->keys = (uint32_t*)(w + L.keys);
+    r->vals = (uint32_t*)(w + L.vals);
+    r->radix_ws = w + L.radix_ws;
+    r->tstart = (uint32_t*)(w + L.tstart);
+    r->tend = (uint32_t*)(w + L.tend);
+    return RF_OK;
+}
 
-    hipLaunchKernelGGL(k_steps, dim3(blocks_for(G.maxw, 64)), dim3(64), 0, s, spectrum, G, table);
+// binner: step tables, quad-step setup, pair expansion, stable sort, tile ranges
+static int bin_impl(const float4* vtx, const float4* bounds, const float* spectrum, const Geom& G,
+                    uint64_t cap, const RasterPtrs& r, hipStream_t s) {
+    const long ntiles = (long)G.ntx * G.nty;
+    hipLaunchKernelGGL(k_steps, dim3(blocks_for(G.maxw, 64)), dim3(64), 0, s, spectrum, G, r.table);
     if (G.NE > 0) {
-        hipLaunchKernelGGL(k_setup, dim3(blocks_for(G.NE, 256)), dim3(256), 0, s, vtx, bounds, table,
-                           G, setup, counts);
+        hipLaunchKernelGGL(k_setup, dim3(blocks_for(G.NE, 256)), dim3(256), 0, s, vtx, bounds,
+                           r.table, G, r.setup, r.counts);
     }
-    int rc = scan_exclusive_u32(counts, offsets, nullptr, (uint64_t)G.NE, &h->required,
-                                w + L.scan_ws, s);
+    int rc = scan_exclusive_u32(r.counts, r.offsets, nullptr, (uint64_t)G.NE, &r.h->required,
+                                r.scan_ws, s);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1), 0, s, h, cap);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1), 0, s, r.h, cap);
     if (G.NE > 0) {
-        hipLaunchKernelGGL(k_expand, dim3(blocks_for(G.NE, 256)), dim3(256), 0, s, setup, counts,
-                           offsets, h, G, keys, vals);
+        hipLaunchKernelGGL(k_expand, dim3(blocks_for(G.NE, 256)), dim3(256), 0, s, r.setup, r.counts,
+                           r.offsets, r.h, G, r.keys, r.vals);
     }
     int bits = 1;
     while ((1L << bits) < ntiles) bits++;
-    rc = radix_sort_pairs(keys, vals, &h->n_sort, cap, bits, w + L.radix_ws, s);
+    rc = radix_sort_pairs(r.keys, r.vals, &r.h->n_sort, cap, bits, r.radix_ws, s);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(ntiles, 256)), dim3(256), 0, s, tstart, ntiles, 0u);
-    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(ntiles, 256)), dim3(256), 0, s, tend, ntiles, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(ntiles, 256)), dim3(256), 0, s, r.tstart, ntiles,
+                       0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(ntiles, 256)), dim3(256), 0, s, r.tend, ntiles, 0u);
     if (cap > 0) {
-        hipLaunchKernelGGL(k_ranges, dim3(blocks_for((long)cap, 256)), dim3(256), 0, s, keys, h,
-                           tstart, tend);
+        hipLaunchKernelGGL(k_ranges, dim3(blocks_for((long)cap, 256)), dim3(256), 0, s, r.keys, r.h,
+                           r.tstart, r.tend);
     }
-    hipLaunchKernelGGL(k_raster, dim3((unsigned)ntiles), dim3(64), 0, s, vtx, bounds, ghost, table,
-                       setup, vals, tstart, tend, h, G, image);
-    RF_LAUNCH_CHECK("rf_rasterize");
+    RF_LAUNCH_CHECK("binner");
     return RF_OK;
+}
+
+static int draw_impl(const float4* vtx, const float4* bounds, const float* ghost, const Geom& G,
+                     const RasterPtrs& r, float4* image, hipStream_t s) {
+    const long ntiles = (long)G.ntx * G.nty;
+    hipLaunchKernelGGL(k_raster, dim3((unsigned)ntiles), dim3(64), 0, s, vtx, bounds, ghost, r.table,
+                       r.setup, r.vals, r.tstart, r.tend, r.h, G, image);
+    RF_LAUNCH_CHECK("rasterizer");
+    return RF_OK;
+}
+
+static int raster_impl(const float4* vtx, const float4* bounds, const float* ghost,
+                       const float* spectrum, const Geom& G, uint64_t cap, void* ws, size_t ws_bytes,
+                       float4* image, hipStream_t s) {
+    RasterPtrs r;
+    int rc = raster_ptrs(G, cap, ws, ws_bytes, &r, "rf_rasterize");
+    if (rc) return rc;
+    rc = bin_impl(vtx, bounds, spectrum, G, cap, r, s);
+    if (rc) return rc;
+    return draw_impl(vtx, bounds, ghost, G, r, image, s);
 }
 
 struct FlareLayout {
@@ -716,6 +750,37 @@ extern "C" int rf_rasterize(const void* vertexes, const float* bounds, const flo
     return raster_impl((const float4*)vertexes, (const float4*)bounds, ghost, light_spectrum, G,
                        pair_capacity, workspace, workspace_bytes, (float4*)image,
                        (hipStream_t)stream);
+}
+
+extern "C" int rf_raster_bin(const void* vertexes, const float* bounds,
+                             const float* light_spectrum, const rf_raster_params* prm,
+                             uint64_t pair_capacity, void* workspace, size_t workspace_bytes,
+                             rf_stream_t stream) {
+    Geom G;
+    int rc = make_geom(prm, &G);
+    if (rc) return rc;
+    if ((G.W > 1 && !light_spectrum) || (G.NE > 0 && (!vertexes || !bounds)))
+        return set_error(RF_ERR_INVALID, "rf_raster_bin: NULL buffer");
+    RasterPtrs r;
+    rc = raster_ptrs(G, pair_capacity, workspace, workspace_bytes, &r, "rf_raster_bin");
+    if (rc) return rc;
+    return bin_impl((const float4*)vertexes, (const float4*)bounds, light_spectrum, G,
+                    pair_capacity, r, (hipStream_t)stream);
+}
+
+extern "C" int rf_raster_draw(const void* vertexes, const float* bounds, const float* ghost,
+                              const rf_raster_params* prm, uint64_t pair_capacity, void* workspace,
+                              size_t workspace_bytes, float* image, rf_stream_t stream) {
+    Geom G;
+    int rc = make_geom(prm, &G);
+    if (rc) return rc;
+    if (!image || !ghost || (G.NE > 0 && (!vertexes || !bounds)))
+        return set_error(RF_ERR_INVALID, "rf_raster_draw: NULL buffer");
+    RasterPtrs r;
+    rc = raster_ptrs(G, pair_capacity, workspace, workspace_bytes, &r, "rf_raster_draw");
+    if (rc) return rc;
+    return draw_impl((const float4*)vertexes, (const float4*)bounds, ghost, G, r, (float4*)image,
+                     (hipStream_t)stream);
 }
 
 extern "C" size_t rf_flare_workspace_size(const rf_raster_params* prm, uint64_t pair_capacity) {

@@ -126,3 +126,29 @@ extern "C" int rf_starburst(const float* spectrum, int spectrum_width, int spect
     RF_LAUNCH_CHECK("rf_starburst");
     return RF_OK;
 }
+
+namespace rf {
+// Engine.flare_starburst (engine.py:174-187): array = flare.copy();
+// array += flare + starburst, i.e. flare + (flare + starburst).
+__global__ __launch_bounds__(256) void k_compose(const float4* __restrict__ flare,
+                                                 const float4* __restrict__ starburst, long n,
+                                                 float4* __restrict__ out) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float4 f = flare[i], s = starburst[i];
+    out[i] = make_float4(f.x + (f.x + s.x), f.y + (f.y + s.y), f.z + (f.z + s.z), f.w + (f.w + s.w));
+}
+}  // namespace rf
+
+extern "C" int rf_compose_flare_starburst(const float* flare, const float* starburst,
+                                          long pixel_count, float* out, rf_stream_t stream) {
+    using namespace rf;
+    if (!flare || !starburst || !out || pixel_count < 0)
+        return set_error(RF_ERR_INVALID, "rf_compose_flare_starburst: bad arguments");
+    if (pixel_count == 0) return RF_OK;
+    hipLaunchKernelGGL(k_compose, dim3((unsigned)((pixel_count + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, (const float4*)flare, (const float4*)starburst,
+                       pixel_count, (float4*)out);
+    RF_LAUNCH_CHECK("rf_compose_flare_starburst");
+    return RF_OK;
+}

@@ -166,8 +166,11 @@ __global__ __launch_bounds__(256) void k_trace(const rf_lens_element* __restrict
         const bool do_reflect =
             !is_stop && !is_sensor && ((step == 0 && lens_id == b1) || (step == 1 && lens_id == b2));
         if (alive) {
-            Hit h = do_reflect ? intersect<true>(pos, dir, radius, center)
-                               : intersect<false>(pos, dir, radius, center);
+            // refraction events never read the incidence angle, except that
+            // the intersection records store it
+            Hit h = (do_reflect || OUT == OUT_INTERSECTIONS)
+                        ? intersect<true>(pos, dir, radius, center)
+                        : intersect<false>(pos, dir, radius, center);
             if (!h.hit) {
                 alive = false;
                 died = true;
