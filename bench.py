@@ -144,6 +144,8 @@ def main() -> int:
     ap.add_argument('--warmup', type=int, default=3)
     ap.add_argument('--workload', default='full', choices=['full', 'small'])
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--copy-calibration', action='store_true',
+                    help='also run a device-to-device copy of 256 MiB (for counter scaling)')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py measures one GPU; --gpus {args.gpus} is not supported (the flare path '
@@ -230,6 +232,12 @@ def main() -> int:
         'trace_tflops': trace_tflops,
         'roofline': roofline,
     }
+    if args.copy_calibration:
+        src = torch.empty((64 << 20,), dtype=torch.float32, device='cuda')
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        line['copy_calibration_bytes'] = 2 * src.numel() * 4
     if not args.no_cpu_baseline:
         print('# cpu baseline (oracle on a bounded sample)...', file=sys.stderr, flush=True)
         line['cpu_baseline'] = cpu_baseline(pipe, args.workload)

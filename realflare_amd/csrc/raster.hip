@@ -51,7 +51,16 @@ struct alignas(16) Setup {
     int vx[4], vy[4];
     int flags;  // bits 0..4: top-left of edges 01,12,23,30,20
     uint32_t r0, r1;  // tile rect (tx0 | ty0 << 16), (tx1 | ty1 << 16)
-    int pad_;
+    uint32_t prim;
+};
+
+// Shading inputs of one (quad, step): the four corners' attributes blended
+// between the step's two wavelengths (mix_vertex, rasterizing.cl:509-522)
+// and the step's XYZ weight.  Evaluated once per quad step instead of once
+// per (tile, quad step).
+struct alignas(16) Attr {
+    float u[4], v[4], rrel[4], refl[4], inten[4];
+    float x, y, z, pad_;
 };
 
 __constant__ unsigned char c_sub_offsets[15] = {0, 1, 0, 1, 2, 0, 3, 4, 1, 6, 2, 5, 0, 3, 7};
@@ -237,11 +246,80 @@ __device__ __forceinline__ void load_pos(const float4* __restrict__ vtx, long vi
     }
 }
 
-// Per (quad, step): integer vertices, edge biases, tile rectangle, pair count.
+// Convex hull of the four integer vertices (sample space), prepared for
+// row queries.  Coverage of the Hormann-Tarini test lies inside triangles
+// (0,1,2) and (2,3,0), hence inside this hull, so tiles it cannot reach need
+// no binning.  A row's x-extent comes from the vertices inside the row's
+// sample band and the crossings of all six vertex pairs with the band's two
+// edges (double precision, widened by one sample on each side).
+struct HullRows {
+    double x[4], y[4];
+    double slope[6];
+    int ia[6], ib[6], n;
+};
+
+__device__ __forceinline__ void hull_prepare(const int* vx, const int* vy, HullRows* h) {
+    int n = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        h->x[i] = vx[i];
+        h->y[i] = vy[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+#pragma unroll
+        for (int j = i + 1; j < 4; j++) {
+            if (vy[i] == vy[j]) continue;
+            h->ia[n] = i;
+            h->ib[n] = j;
+            h->slope[n] = ((double)vx[j] - vx[i]) / ((double)vy[j] - vy[i]);
+            n++;
+        }
+    }
+    h->n = n;
+}
+
+// Tiles [*a, *b] of tile row `ty` the hull can reach (empty when *a > *b).
+__device__ __forceinline__ void hull_row_span(const HullRows& h, int AA, int ty, long* a, long* b) {
+    const double Y0 = (double)ty * TILE * AA, Y1 = ((double)ty + 1) * TILE * AA - 1;
+    double xmin = INFINITY, xmax = -INFINITY;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        if (h.y[i] >= Y0 && h.y[i] <= Y1) {
+            xmin = fmin(xmin, h.x[i]);
+            xmax = fmax(xmax, h.x[i]);
+        }
+    }
+    for (int s = 0; s < h.n; s++) {
+        const double xa = h.x[h.ia[s]], ya = h.y[h.ia[s]], yb = h.y[h.ib[s]];
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const double Y = k ? Y1 : Y0;
+            if ((ya - Y) * (yb - Y) <= 0) {
+                const double x = xa + (Y - ya) * h.slope[s];
+                xmin = fmin(xmin, x);
+                xmax = fmax(xmax, x);
+            }
+        }
+    }
+    if (!(xmin <= xmax)) {
+        *a = 1;
+        *b = 0;
+        return;
+    }
+    const double lim = 4.0e18;
+    const long sl = (long)fmax(fmin(floor(xmin) - 1, lim), -lim);
+    const long sh = (long)fmax(fmin(ceil(xmax) + 1, lim), -lim);
+    *a = floordiv(floordiv(sl, AA), TILE);
+    *b = floordiv(floordiv(sh, AA), TILE);
+}
+
+// Per (quad, step): integer vertices, edge biases, tile rectangle, pair
+// count, and the blended shading attributes.
 __global__ __launch_bounds__(256) void k_setup(const float4* __restrict__ vtx,
                                                const float4* __restrict__ bounds,
                                                const float4* __restrict__ table, Geom G,
-                                               Setup* __restrict__ setup,
+                                               Setup* __restrict__ setup, Attr* __restrict__ attr,
                                                uint32_t* __restrict__ counts) {
 #pragma clang fp contract(off)
     const long e = (long)blockIdx.x * 256 + threadIdx.x;
@@ -254,27 +332,42 @@ __global__ __launch_bounds__(256) void k_setup(const float4* __restrict__ vtx,
     Setup rec;
     rec.flags = 0;
     rec.r0 = rec.r1 = 0;
-    rec.pad_ = 0;
+    rec.prim = (uint32_t)prim;
     for (int j = 0; j < 4; j++) rec.vx[j] = rec.vy[j] = 0;
     if (!isnan(b.x)) {
         const int path = (int)(prim / G.Q), quad = (int)(prim % G.Q);
         int qv[4];
         quad_vertexes(G.g, quad, qv);
-        const float blend = table[step].x;
+        const float4 tab = table[step];
+        const float blend = tab.x;
         int vx[4], vy[4];
+        Attr at;
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const long base = ((long)path * G.R + qv[j]) * G.W + wl;
-            float ax, ay, bx, by;
-            load_pos(vtx, base, G.NV, ax, ay);
-            load_pos(vtx, base + 1, G.NV, bx, by);
-            float mx = ax + (bx - ax) * blend;
-            float my = ay + (by - ay) * blend;
+            const float4 a0 = vtx[2 * base], a1 = vtx[2 * base + 1];
+            float4 b0 = make_float4(0, 0, 0, 0), b1 = make_float4(0, 0, 0, 0);
+            if (base + 1 < G.NV) {
+                b0 = vtx[2 * (base + 1)];
+                b1 = vtx[2 * (base + 1) + 1];
+            }
+            const float mx = a0.x + (b0.x - a0.x) * blend;
+            const float my = a0.y + (b0.y - a0.y) * blend;
             vx[j] = rf_to_int(mx * (float)G.AA);
             vy[j] = rf_to_int(my * (float)G.AA);
             rec.vx[j] = vx[j];
             rec.vy[j] = vy[j];
+            at.u[j] = a0.z + (b0.z - a0.z) * blend;
+            at.v[j] = a0.w + (b0.w - a0.w) * blend;
+            at.rrel[j] = a1.x + (b1.x - a1.x) * blend;
+            at.refl[j] = a1.y + (b1.y - a1.y) * blend;
+            at.inten[j] = a1.z + (b1.z - a1.z) * blend;
         }
+        at.x = tab.y;
+        at.y = tab.z;
+        at.z = tab.w;
+        at.pad_ = 0.0f;
+        attr[e] = at;
         const bool collinear = edge_i(vx[0], vy[0], vx[1], vy[1], vx[2], vy[2]) == 0 &&
                                edge_i(vx[0], vy[0], vx[2], vy[2], vx[3], vy[3]) == 0;
         if (!collinear) {
@@ -312,7 +405,15 @@ __global__ __launch_bounds__(256) void k_setup(const float4* __restrict__ vtx,
                 const uint32_t ty0 = (uint32_t)(y0 / TILE), ty1 = (uint32_t)(y1 / TILE);
                 rec.r0 = tx0 | (ty0 << 16);
                 rec.r1 = tx1 | (ty1 << 16);
-                count = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
+                HullRows hull;
+                hull_prepare(vx, vy, &hull);
+                for (uint32_t ty = ty0; ty <= ty1; ty++) {
+                    long a, bb;
+                    hull_row_span(hull, G.AA, (int)ty, &a, &bb);
+                    a = max(a, (long)tx0);
+                    bb = min(bb, (long)tx1);
+                    if (a <= bb) count += (uint32_t)(bb - a + 1);
+                }
             }
         }
     }
@@ -326,6 +427,9 @@ __global__ void k_finalize(Header* __restrict__ h, uint64_t capacity) {
     h->n_sort = need > capacity ? 0 : need;
 }
 
+// One thread per (quad, step): walks the tile rows of its rectangle with the
+// hull span k_setup counted with and writes its (tile, entry) pairs at its
+// offset from the scan.
 __global__ __launch_bounds__(256) void k_expand(const Setup* __restrict__ setup,
                                                 const uint32_t* __restrict__ counts,
                                                 const uint64_t* __restrict__ offsets,
@@ -334,17 +438,23 @@ __global__ __launch_bounds__(256) void k_expand(const Setup* __restrict__ setup,
                                                 uint32_t* __restrict__ vals) {
     const long e = (long)blockIdx.x * 256 + threadIdx.x;
     if (e >= G.NE || h->overflow) return;
-    const uint32_t count = counts[e];
-    if (!count) return;
-    const Setup& rec = setup[e];
+    if (!counts[e]) return;
+    const Setup rec = setup[e];
     const uint32_t tx0 = rec.r0 & 0xffff, ty0 = rec.r0 >> 16;
-    const uint32_t tx1 = rec.r1 & 0xffff;
-    const uint32_t rw = tx1 - tx0 + 1;
-    const uint64_t off = offsets[e];
-    for (uint32_t k = 0; k < count; k++) {
-        const uint32_t tx = tx0 + k % rw, ty = ty0 + k / rw;
-        keys[off + k] = ty * (uint32_t)G.ntx + tx;
-        vals[off + k] = (uint32_t)e;
+    const uint32_t tx1 = rec.r1 & 0xffff, ty1 = rec.r1 >> 16;
+    HullRows hull;
+    hull_prepare(rec.vx, rec.vy, &hull);
+    uint64_t off = offsets[e];
+    for (uint32_t ty = ty0; ty <= ty1; ty++) {
+        long a, b;
+        hull_row_span(hull, G.AA, (int)ty, &a, &b);
+        a = max(a, (long)tx0);
+        b = min(b, (long)tx1);
+        for (long tx = a; tx <= b; tx++) {
+            keys[off] = ty * (uint32_t)G.ntx + (uint32_t)tx;
+            vals[off] = (uint32_t)e;
+            off++;
+        }
     }
 }
 
@@ -418,11 +528,56 @@ __device__ __forceinline__ void barycentric_quad(int px, int py, const int* x, c
     for (int i = 0; i < 4; i++) w[i] = ww[i] / sum;
 }
 
-__global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ vtx,
-                                               const float4* __restrict__ bounds,
+// Division and square root of the shading math at the precision OpenCL
+// guarantees for them (x/y within 2.5 ulp, sqrt within 3 ulp): a hardware
+// reciprocal or square root rounded once more.  The integer coverage test
+// does not use them.
+__device__ __forceinline__ float cl_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ float cl_sqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
+
+// rasterizing.cl:95-148 (mean value coordinates of a quad), shading precision
+__device__ __forceinline__ void barycentric_fast(int px, int py, const int* x, const int* y,
+                                                 float* w) {
+#pragma clang fp contract(off)
+    float sx[4], sy[4], a[4], d[4], r[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        sx[i] = (float)wsub(x[i], px);
+        sy[i] = (float)wsub(y[i], py);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int j = (i + 1) & 3;
+        a[i] = sx[i] * sy[j] - sx[j] * sy[i];
+        d[i] = sx[i] * sx[j] + sy[i] * sy[j];
+        r[i] = cl_sqrt(sx[i] * sx[i] + sy[i] * sy[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        if (r[i] == 0) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) w[k] = (k == i) ? 1.0f : 0.0f;
+            return;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+        if (a[i] == 0) a[i] = 1e-3f;
+    float t[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) t[i] = cl_div(r[i] * r[(i + 1) & 3] - d[i], a[i]);
+    float ww[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) ww[i] = cl_div(t[(i + 3) & 3] + t[i], r[i]);
+    const float inv = __builtin_amdgcn_rcpf(ww[0] + ww[1] + ww[2] + ww[3]);
+#pragma unroll
+    for (int i = 0; i < 4; i++) w[i] = ww[i] * inv;
+}
+
+__global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ bounds,
                                                const float* __restrict__ ghost,
-                                               const float4* __restrict__ table,
                                                const Setup* __restrict__ setup,
+                                               const Attr* __restrict__ attr,
                                                const uint32_t* __restrict__ vals,
                                                const uint32_t* __restrict__ tstart,
                                                const uint32_t* __restrict__ tend,
@@ -438,16 +593,16 @@ __global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ vtx,
     const int px = x * G.AA, py = y * G.AA;
     const int pcx = px + G.AA / 2, pcy = py + G.AA / 2;
     const int bxp = x / G.bin, byp = y / G.bin;
+    const float inv_scale = 1.0f / G.ghost_scale;
     float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f;
     const uint32_t start = tstart[tile], end = tend[tile];
     for (uint32_t k = start; k < end; k++) {
         const uint32_t e = vals[k];
         const Setup rec = setup[e];
-        const long prim = e / G.S;
         int hits = 0;
         bool ok = inside;
         if (ok && !G.bin_exact) {
-            const float4 b = bounds[prim];
+            const float4 b = bounds[rec.prim];
             const int sbx0 = wadd(rf_to_int(b.x * G.st), G.width / 2);
             const int sby0 = wadd(rf_to_int(b.y * G.st), G.height / 2);
             const int sbx1 = wadd(rf_to_int(b.z * G.st), G.width / 2);
@@ -462,48 +617,31 @@ __global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ vtx,
             }
         }
         if (!__any(hits > 0)) continue;
-        const int step = (int)(e % G.S);
-        const int wl = step / G.Wsub;
-        const float4 tab = table[step];
-        const float blend = tab.x;
-        const int path = (int)(prim / G.Q), quad = (int)(prim % G.Q);
-        int qv[4];
-        quad_vertexes(G.g, quad, qv);
-        float u[4], v[4], rr[4], rf[4], it[4];
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const long base = ((long)path * G.R + qv[j]) * G.W + wl;
-            const float4 a0 = vtx[2 * base], a1 = vtx[2 * base + 1];
-            float4 b0 = make_float4(0, 0, 0, 0), b1 = make_float4(0, 0, 0, 0);
-            if (base + 1 < G.NV) {
-                b0 = vtx[2 * (base + 1)];
-                b1 = vtx[2 * (base + 1) + 1];
-            }
-            u[j] = a0.z + (b0.z - a0.z) * blend;
-            v[j] = a0.w + (b0.w - a0.w) * blend;
-            rr[j] = a1.x + (b1.x - a1.x) * blend;
-            rf[j] = a1.y + (b1.y - a1.y) * blend;
-            it[j] = a1.z + (b1.z - a1.z) * blend;
-        }
+        const Attr at = attr[e];
         if (hits > 0) {
             float w[4];
-            barycentric_quad(pcx, pcy, rec.vx, rec.vy, w);
+            barycentric_fast(pcx, pcy, rec.vx, rec.vy, w);
             // fragment_shader, rasterizing.cl:471-506
-            float uu = w[0] * u[0] + w[1] * u[1] + w[2] * u[2] + w[3] * u[3];
-            float vv = w[0] * v[0] + w[1] * v[1] + w[2] * v[2] + w[3] * v[3];
-            uu = (uu / G.ghost_scale + 1.0f) / 2.0f;
-            vv = (vv / G.ghost_scale + 1.0f) / 2.0f;
+            float uu = w[0] * at.u[0] + w[1] * at.u[1] + w[2] * at.u[2] + w[3] * at.u[3];
+            float vv = w[0] * at.v[0] + w[1] * at.v[1] + w[2] * at.v[2] + w[3] * at.v[3];
+            uu = (uu * inv_scale + 1.0f) * 0.5f;
+            vv = (vv * inv_scale + 1.0f) * 0.5f;
             const float gi = sample_bilinear(ghost, G.gw, G.gh, 1, uu, vv, false);
-            const float rel = w[0] * rr[0] + w[1] * rr[1] + w[2] * rr[2] + w[3] * rr[3];
-            const float ri = smoothstepf(1.0f, 0.95f, rel);
-            float ai = w[0] * it[0] + w[1] * it[1] + w[2] * it[2] + w[3] * it[3];
-            float ci = w[0] * rf[0] + w[1] * rf[1] + w[2] * rf[2] + w[3] * rf[3];
+            const float rel = w[0] * at.rrel[0] + w[1] * at.rrel[1] + w[2] * at.rrel[2] +
+                              w[3] * at.rrel[3];
+            // smoothstep(1, 0.95, rrel)
+            const float tt = clampn(cl_div(rel - 1.0f, 0.95f - 1.0f), 0.0f, 1.0f);
+            const float ri = tt * tt * (3.0f - 2.0f * tt);
+            float ai = w[0] * at.inten[0] + w[1] * at.inten[1] + w[2] * at.inten[2] +
+                       w[3] * at.inten[3];
+            float ci = w[0] * at.refl[0] + w[1] * at.refl[1] + w[2] * at.refl[2] +
+                       w[3] * at.refl[3];
             ai = fmaxf(ai, 0.0f);
             ci = fmaxf(ci, 0.0f);
             const float frag = 0.0f + gi * ri * ai * ci * (float)hits;
-            r0 += tab.y * frag;
-            r1 += tab.z * frag;
-            r2 += tab.w * frag;
+            r0 += at.x * frag;
+            r1 += at.y * frag;
+            r2 += at.z * frag;
         }
     }
     if (!inside) return;
@@ -572,7 +710,8 @@ static int make_geom(const rf_raster_params* p, Geom* G) {
 static inline unsigned blocks_for(long n, int threads) { return (unsigned)((n + threads - 1) / threads); }
 
 struct Layout {
-    size_t header, table, setup, counts, offsets, scan_ws, keys, vals, radix_ws, tstart, tend, total;
+    size_t header, table, setup, attr, counts, offsets, scan_ws, keys, vals, radix_ws, tstart, tend,
+        total;
 };
 
 static Layout layout(const Geom& G, uint64_t cap) {
@@ -586,6 +725,7 @@ static Layout layout(const Geom& G, uint64_t cap) {
     L.header = take(sizeof(Header));
     L.table = take((size_t)G.S * sizeof(float4));
     L.setup = take((size_t)G.NE * sizeof(Setup));
+    L.attr = take((size_t)G.NE * sizeof(Attr));
     L.counts = take((size_t)G.NE * sizeof(uint32_t));
     L.offsets = take((size_t)G.NE * sizeof(uint64_t));
     L.scan_ws = take(scan_workspace_bytes((uint64_t)G.NE));
@@ -603,6 +743,7 @@ struct RasterPtrs {
     Header* h;
     float4* table;
     Setup* setup;
+    Attr* attr;
     uint32_t *counts, *keys, *vals, *tstart, *tend;
     uint64_t* offsets;
     void *scan_ws, *radix_ws;
@@ -618,6 +759,7 @@ static int raster_ptrs(const Geom& G, uint64_t cap, void* ws, size_t ws_bytes, R
     r->h = (Header*)(w + L.header);
     r->table = (float4*)(w + L.table);
     r->setup = (Setup*)(w + L.setup);
+    r->attr = (Attr*)(w + L.attr);
     r->counts = (uint32_t*)(w + L.counts);
     r->offsets = (uint64_t*)(w + L.offsets);
     r->scan_ws = w + L.scan_ws;
@@ -636,7 +778,7 @@ static int bin_impl(const float4* vtx, const float4* bounds, const float* spectr
     hipLaunchKernelGGL(k_steps, dim3(blocks_for(G.maxw, 64)), dim3(64), 0, s, spectrum, G, r.table);
     if (G.NE > 0) {
         hipLaunchKernelGGL(k_setup, dim3(blocks_for(G.NE, 256)), dim3(256), 0, s, vtx, bounds,
-                           r.table, G, r.setup, r.counts);
+                           r.table, G, r.setup, r.attr, r.counts);
     }
     int rc = scan_exclusive_u32(r.counts, r.offsets, nullptr, (uint64_t)G.NE, &r.h->required,
                                 r.scan_ws, s);
@@ -664,8 +806,9 @@ static int bin_impl(const float4* vtx, const float4* bounds, const float* spectr
 static int draw_impl(const float4* vtx, const float4* bounds, const float* ghost, const Geom& G,
                      const RasterPtrs& r, float4* image, hipStream_t s) {
     const long ntiles = (long)G.ntx * G.nty;
-    hipLaunchKernelGGL(k_raster, dim3((unsigned)ntiles), dim3(64), 0, s, vtx, bounds, ghost, r.table,
-                       r.setup, r.vals, r.tstart, r.tend, r.h, G, image);
+    (void)vtx;
+    hipLaunchKernelGGL(k_raster, dim3((unsigned)ntiles), dim3(64), 0, s, bounds, ghost, r.setup,
+                       r.attr, r.vals, r.tstart, r.tend, r.h, G, image);
     RF_LAUNCH_CHECK("rasterizer");
     return RF_OK;
 }
