@@ -137,6 +137,74 @@ def cpu_baseline(pipe, inp_name: str = 'full') -> dict:
     }
 
 
+def adjoint_timing(wl, steps: int) -> dict:
+    """Differentiable configuration: the same 12-surface lens, 66 ghosts,
+    64x64 rays, 8 wavelengths, upstream gradients of ones.  Device time per
+    batch of: the plain forward trace (rf_trace_planes_coated), the forward
+    pass that also writes the checkpoint tape, the backward pass from that
+    tape, and forward + backward fused in one launch."""
+    import ctypes
+
+    import torch
+
+    from realflare_amd import native
+    from realflare_amd.adjoint import TraceSpec, lens_params
+    from realflare_amd.api.tasks.raytracing import light_direction
+
+    p = wl.project
+    sensor = (p.flare.lens.sensor_size.width(), p.flare.lens.sensor_size.height())
+    res = (p.render.resolution.width(), p.render.resolution.height())
+    params, static = lens_params(wl.lens_model, sensor, p.flare.lens.glasses_path,
+                                 p.flare.lens.abbe_nr_adjustment, p.flare.lens.coating)
+    direction = light_direction((p.flare.light.position.x(), p.flare.light.position.y()), res,
+                                sensor, wl.lens_model.focal_length)
+    out = {}
+    for K in (1, 4):
+        spec = TraceSpec(static, wl.paths, wl.wavelengths, p.flare.lens.coating_min_ior, 64,
+                         p.render.grid_length, direction, checkpoint_every=K)
+        ones = torch.ones(spec.shape, dtype=torch.float32, device='cuda')
+        lens, coat = spec.upload(params)
+        P, W, R = spec.shape
+        planes = torch.empty((6, P, W, R), dtype=torch.float32, device='cuda')
+        fwd_out = torch.empty((3, P, W, R), dtype=torch.float32, device='cuda')
+        tape = torch.empty((spec.ws_bytes,), dtype=torch.uint8, device='cuda')
+        gout = torch.empty((3 * (static.n_model + 1) * (1 + W),), dtype=torch.float32, device='cuda')
+        common = (native.ptr(lens), native.ptr(coat), native.ptr(spec.paths_t),
+                  native.ptr(spec.wl_t), ctypes.byref(spec.prm))
+        grads = (native.ptr(ones), native.ptr(ones), native.ptr(ones))
+        ws = (native.ptr(tape), ctypes.c_size_t(spec.ws_bytes))
+        fns = {
+            'forward': lambda: native.call('rf_trace_planes_coated', *common, native.ptr(planes),
+                                           native.stream()),
+            'forward_checkpointed': lambda: native.call(
+                'rf_trace_forward_checkpointed', *common, spec.K, *ws, native.ptr(fwd_out),
+                native.stream()),
+            'backward': lambda: native.call('rf_trace_backward', *common, spec.K, *grads, *ws,
+                                            native.ptr(gout), native.stream()),
+            'fused': lambda: native.call('rf_trace_adjoint', *common, spec.K, *grads, *ws,
+                                         native.ptr(gout), None, native.stream()),
+        }
+        times = {}
+        for name in ('forward', 'forward_checkpointed', 'backward', 'fused'):
+            fn = fns[name]
+            if name == 'backward':
+                fns['forward_checkpointed']()
+            fn()
+            torch.cuda.synchronize()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            for _ in range(steps):
+                fn()
+            b.record()
+            torch.cuda.synchronize()
+            times[name + '_ms'] = a.elapsed_time(b) / steps
+        times['backward_over_forward'] = times['backward_ms'] / times['forward_ms']
+        times['fused_over_forward'] = times['fused_ms'] / times['forward_ms']
+        out[f'K{K}'] = times
+    out['rays'] = 66 * 8 * 64 * 64
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -232,6 +300,9 @@ def main() -> int:
         'trace_tflops': trace_tflops,
         'roofline': roofline,
     }
+    if args.workload == 'full':
+        line['adjoint'] = adjoint_timing(wl, args.steps)
+        print(f"# adjoint: {line['adjoint']}", file=sys.stderr, flush=True)
     if args.copy_calibration:
         src = torch.empty((64 << 20,), dtype=torch.float32, device='cuda')
         dst = torch.empty_like(src)

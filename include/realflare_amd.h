@@ -88,6 +88,58 @@ int rf_trace_intersections(const rf_lens_element *lens, const int32_t *paths,
                            const int32_t *wavelengths, const rf_trace_params *prm, void *rays,
                            void *intersections, rf_stream_t stream);
 
+/*
+ * rf_trace_planes with the coating wavelength of every element taken from
+ * coating_lambda[lens_count] (float, nm) instead of the element records, so
+ * the coating is a continuous parameter.  Same kernel as rf_trace_planes; the
+ * forward pass of the differentiable trace.  The coating value is the
+ * element's `coating` field that raytracing.cl:310-318 reads.
+ */
+int rf_trace_planes_coated(const rf_lens_element *lens, const float *coating_lambda,
+                           const int32_t *paths, const int32_t *wavelengths,
+                           const rf_trace_params *prm, float *planes, rf_stream_t stream);
+
+/*
+ * Adjoint (reverse mode) of the ghost trace.  No counterpart in the
+ * reference, which has no gradients: it differentiates the `raytrace` kernel
+ * (raytracing.cl:218-359) as rf_trace_planes_coated evaluates it.
+ *   grad_pos_x, grad_pos_y, grad_reflectance   float[P*W*R] upstream gradients
+ *     of the sensor position and reflectance of every ray (ignored for rays
+ *     that do not reach the sensor)
+ *   grad_out   float[3L + W*L]: d/d radius[L], d/d centre[L] (element.center,
+ *     raytracing.cl:104-168), d/d coating_lambda[L], then d/d ior[w][L], the
+ *     refractive index each element has at wavelength w
+ *   forward_out  optional float[3][P*W*R]: x, y, reflectance of the forward
+ *     pass (NaN where the ray does not reach the sensor)
+ *   checkpoint_every  K in {1, 2, 3, 4, 6, 8}: ray state is stored every K
+ *     surface events and the events in between are recomputed; K does not
+ *     change the result
+ * The result is deterministic.  rf_trace_adjoint runs the forward pass and the
+ * reverse pass in one launch; rf_trace_forward_checkpointed followed by
+ * rf_trace_backward splits them (the forward pass leaves its checkpoints in
+ * the workspace, which must be passed unchanged, with the same arguments, to
+ * rf_trace_backward) and gives bit-identical results.
+ */
+size_t rf_trace_adjoint_workspace_size(const rf_trace_params *prm, int checkpoint_every);
+int rf_trace_adjoint(const rf_lens_element *lens, const float *coating_lambda,
+                     const int32_t *paths, const int32_t *wavelengths,
+                     const rf_trace_params *prm, int checkpoint_every, const float *grad_pos_x,
+                     const float *grad_pos_y, const float *grad_reflectance, void *workspace,
+                     size_t workspace_bytes, float *grad_out, float *forward_out,
+                     rf_stream_t stream);
+/* Forward half: forward_out (required) as above, checkpoints into workspace. */
+int rf_trace_forward_checkpointed(const rf_lens_element *lens, const float *coating_lambda,
+                                  const int32_t *paths, const int32_t *wavelengths,
+                                  const rf_trace_params *prm, int checkpoint_every,
+                                  void *workspace, size_t workspace_bytes, float *forward_out,
+                                  rf_stream_t stream);
+/* Reverse half, from the checkpoints rf_trace_forward_checkpointed left. */
+int rf_trace_backward(const rf_lens_element *lens, const float *coating_lambda,
+                      const int32_t *paths, const int32_t *wavelengths,
+                      const rf_trace_params *prm, int checkpoint_every, const float *grad_pos_x,
+                      const float *grad_pos_y, const float *grad_reflectance, void *workspace,
+                      size_t workspace_bytes, float *grad_out, rf_stream_t stream);
+
 /* Settings of the rasterizer kernels (rasterizing.cl:233-663), as
  * RasterizingTask.rasterize derives them (rasterizing.py:296-421). */
 typedef struct rf_raster_params {

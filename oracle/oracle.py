@@ -396,3 +396,27 @@ def sample_bilinear(tex, s, t, edge=False, precision='f32') -> float:
                 [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                  ctypes.c_int])
     return f(tex.ctypes.data, tex.shape[1], tex.shape[0], s, t, int(edge))
+
+
+def trace_params(radius, ior, height, center, coating, coefficients, disperse, paths, wavelengths,
+                 aperture_index, coating_min_ior, grid_count, grid_length, direction,
+                 precision='f64'):
+    """Trace from element parameters given in double ([L] arrays, coefficients
+    [L][6]); used for finite differences of the trace."""
+    a = [np.ascontiguousarray(np.asarray(v, np.float64)) for v in
+         (radius, ior, height, center, coating)]
+    coef = np.ascontiguousarray(np.asarray(coefficients, np.float64).reshape(-1, 6))
+    L = len(a[0])
+    paths = np.ascontiguousarray(np.asarray(paths, np.int32).reshape(-1, 2))
+    wavelengths = np.ascontiguousarray(np.asarray(wavelengths, np.int32))
+    P, W, R = len(paths), len(wavelengths), grid_count * grid_count
+    out = np.zeros((P, W, R), ray_dtype(precision))
+    direction = np.ascontiguousarray(np.asarray(direction, np.float64).reshape(4))
+    f = _fn('rfo_trace_params', precision)
+    f.restype = ctypes.c_int
+    rc = f(*[_p(x) for x in a], _p(coef), ctypes.c_int(L), ctypes.c_int(int(disperse)), _p(paths),
+           ctypes.c_int(P), _p(wavelengths), ctypes.c_int(W), ctypes.c_int(aperture_index),
+           ctypes.c_double(coating_min_ior), ctypes.c_int(grid_count),
+           ctypes.c_double(grid_length), _p(direction), _p(out))
+    assert rc == 0
+    return out

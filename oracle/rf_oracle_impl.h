@@ -49,6 +49,14 @@ typedef struct {
 } FN(vertex_t);
 #define VERTEX FN(vertex_t)
 
+/* A lens element in the working precision (the f64 build takes double
+ * parameters so finite differences can resolve small steps). */
+typedef struct {
+    REAL radius, ior, height, center, coating;
+    REAL coefficients[6];
+} FN(elem_t);
+#define ELEM FN(elem_t)
+
 static inline V3 FN(v3)(REAL x, REAL y, REAL z) { V3 r = {x, y, z}; return r; }
 static inline V3 FN(v3_add)(V3 a, V3 b) { return FN(v3)(a.x + b.x, a.y + b.y, a.z + b.z); }
 static inline V3 FN(v3_sub)(V3 a, V3 b) { return FN(v3)(a.x - b.x, a.y - b.y, a.z - b.z); }
@@ -112,20 +120,20 @@ static V3 FN(reflect)(V3 i, V3 n) {
 }
 
 /* raytracing.cl:107-120 (lambda in nm; `lambda * 1e-3` is a double product) */
-static REAL FN(dispersion)(REAL lambda, const float *c) {
+static REAL FN(dispersion)(REAL lambda, const REAL *c) {
     REAL l = (REAL)((double)lambda * 1e-3);
     REAL l2 = l * l;
-    REAL d0 = ((REAL)c[0] * l2) / (l2 - (REAL)c[1]);
-    REAL d1 = ((REAL)c[2] * l2) / (l2 - (REAL)c[3]);
-    REAL d2 = ((REAL)c[4] * l2) / (l2 - (REAL)c[5]);
+    REAL d0 = (c[0] * l2) / (l2 - c[1]);
+    REAL d1 = (c[2] * l2) / (l2 - c[3]);
+    REAL d2 = (c[4] * l2) / (l2 - c[5]);
     return SQRT(1 + d0 + d1 + d2);
 }
 
 /* raytracing.cl:123-175 */
-static INTER FN(intersect)(RAY ray, const rf_lens_element_t *lens) {
+static INTER FN(intersect)(RAY ray, const ELEM *lens) {
     INTER i;
     memset(&i, 0, sizeof(i));
-    REAL radius = (REAL)lens->radius, center = (REAL)lens->center;
+    REAL radius = lens->radius, center = lens->center;
     if (ray.dir.z == 0) {
         i.hit = 0;
     } else if (radius == 0) {
@@ -158,7 +166,7 @@ static INTER FN(intersect)(RAY ray, const rf_lens_element_t *lens) {
 }
 
 /* raytracing.cl:178-215 */
-static RAY FN(init_ray)(const rf_lens_element_t *lens, int ray_id, int grid_count,
+static RAY FN(init_ray)(const ELEM *lens, int ray_id, int grid_count,
                         REAL grid_length, const REAL *initial_direction) {
     int y = (int)TRUNC((REAL)ray_id / (REAL)grid_count);
     int x = ray_id - (y * grid_count);
@@ -182,28 +190,27 @@ static RAY FN(init_ray)(const rf_lens_element_t *lens, int ray_id, int grid_coun
 
 /* One lane of raytracing.cl:218-359.  `inters` (may be NULL) receives one
  * record per visited surface, as the STORE_INTERSECTIONS build does. */
-static RAY FN(trace_one)(const rf_lens_element_t *lens, int L, int px, int py, REAL wavelength,
-                         int aperture_index, REAL coating_min_ior, int grid_count,
-                         REAL grid_length, const REAL *direction, int ray_id,
+static RAY FN(trace_one)(const ELEM *lens, int L, int disperse, int px, int py,
+                         REAL wavelength, int aperture_index, REAL coating_min_ior,
+                         int grid_count, REAL grid_length, const REAL *direction, int ray_id,
                          INTER *inters, int inters_count) {
     RAY ray = FN(init_ray)(&lens[0], ray_id, grid_count, grid_length, direction);
     int step = 0, inter_id = 0, delta = 1;
-    int disperse = !isnan(lens[0].coefficients[0]);
     /* size_t in the source: a stop listed as a bounce walks the index below 0,
      * which wraps and ends the loop. */
     long lens_id;
     for (lens_id = 0; lens_id >= 0 && lens_id < L; inter_id++, lens_id += delta) {
-        const rf_lens_element_t *el = &lens[lens_id];
+        const ELEM *el = &lens[lens_id];
         INTER inter = FN(intersect)(ray, el);
         if (!inter.hit) break;
         if (inters && inter_id < inters_count) inters[inter_id] = inter;
         ray.pos = inter.pos;
         if (lens_id == aperture_index) {
-            ray.pos_apt.x = inter.pos.x / (REAL)el->height;
-            ray.pos_apt.y = inter.pos.y / (REAL)el->height;
+            ray.pos_apt.x = inter.pos.x / el->height;
+            ray.pos_apt.y = inter.pos.y / el->height;
             continue;
         } else {
-            REAL h = SQRT(inter.pos.x * inter.pos.x + inter.pos.y * inter.pos.y) / (REAL)el->height;
+            REAL h = SQRT(inter.pos.x * inter.pos.x + inter.pos.y * inter.pos.y) / el->height;
             ray.rrel = FN(fmaxn)(ray.rrel, h);
         }
         if (lens_id == L - 1) continue;
@@ -214,8 +221,8 @@ static RAY FN(trace_one)(const rf_lens_element_t *lens, int L, int px, int py, R
             delta = -delta;
         }
         long n_index = (ray.dir.z < 0) ? lens_id - 1 : lens_id + 1;
-        REAL n1 = (0 <= n_index && n_index < L) ? (REAL)lens[n_index].ior : 1;
-        REAL n2 = (REAL)el->ior;
+        REAL n1 = (0 <= n_index && n_index < L) ? lens[n_index].ior : 1;
+        REAL n2 = el->ior;
         if (disperse && n1 > 1) n1 = FN(dispersion)(wavelength, lens[n_index].coefficients);
         if (disperse && n2 > 1) n2 = FN(dispersion)(wavelength, el->coefficients);
         if (do_reflect) {
@@ -223,7 +230,7 @@ static RAY FN(trace_one)(const rf_lens_element_t *lens, int L, int px, int py, R
             REAL theta = (REAL)((double)inter.incident + 1e-9);
             REAL reflectance;
             if (el->coating > 0) {
-                REAL lambda = (REAL)el->coating;
+                REAL lambda = el->coating;
                 REAL nc = FN(fmaxn)(SQRT(n1 * n2), coating_min_ior);
                 REAL d1 = lambda / ((REAL)4.0 * nc);
                 reflectance = FN(fresnel_ar)(theta, wavelength, d1, n1, nc, n2);
@@ -241,11 +248,10 @@ static RAY FN(trace_one)(const rf_lens_element_t *lens, int L, int px, int py, R
     return ray;
 }
 
-/* Whole NDRange of `raytrace`: out[P][W][R] rays. */
-int FN(rfo_trace)(const rf_lens_element_t *lens, int L, const int *paths, int P,
-                  const int *wavelengths, int W, int aperture_index, double coating_min_ior,
-                  int grid_count, double grid_length, const double *direction, RAY *out,
-                  INTER *inters, int inters_count) {
+static int FN(trace_elems)(const ELEM *el, int L, int disperse, const int *paths, int P,
+                           const int *wavelengths, int W, int aperture_index,
+                           double coating_min_ior, int grid_count, double grid_length,
+                           const double *direction, RAY *out, INTER *inters, int inters_count) {
     int R = grid_count * grid_count;
     REAL dir[4] = {(REAL)(float)direction[0], (REAL)(float)direction[1],
                    (REAL)(float)direction[2], (REAL)(float)direction[3]};
@@ -258,11 +264,54 @@ int FN(rfo_trace)(const rf_lens_element_t *lens, int L, const int *paths, int P,
         int w = (int)((idx / R) % W);
         int p = (int)(idx / ((long)R * W));
         INTER *rec = inters ? inters + idx * inters_count : NULL;
-        out[idx] = FN(trace_one)(lens, L, paths[2 * p], paths[2 * p + 1], (REAL)wavelengths[w],
-                                 aperture_index, cmi, grid_count, gl, dir, ray_id, rec,
-                                 inters_count);
+        out[idx] = FN(trace_one)(el, L, disperse, paths[2 * p], paths[2 * p + 1],
+                                 (REAL)wavelengths[w], aperture_index, cmi, grid_count, gl, dir,
+                                 ray_id, rec, inters_count);
     }
     return 0;
+}
+
+/* Whole NDRange of `raytrace`: out[P][W][R] rays.  `coating` (may be NULL)
+ * overrides the elements' integer coating wavelengths. */
+int FN(rfo_trace)(const rf_lens_element_t *lens, int L, const int *paths, int P,
+                  const int *wavelengths, int W, int aperture_index, double coating_min_ior,
+                  int grid_count, double grid_length, const double *direction, RAY *out,
+                  INTER *inters, int inters_count) {
+    ELEM el[256];
+    if (L > 256) return -1;
+    for (int i = 0; i < L; i++) {
+        el[i].radius = (REAL)lens[i].radius;
+        el[i].ior = (REAL)lens[i].ior;
+        el[i].height = (REAL)lens[i].height;
+        el[i].center = (REAL)lens[i].center;
+        el[i].coating = (REAL)lens[i].coating;
+        for (int k = 0; k < 6; k++) el[i].coefficients[k] = (REAL)lens[i].coefficients[k];
+    }
+    int disperse = !isnan(lens[0].coefficients[0]);
+    return FN(trace_elems)(el, L, disperse, paths, P, wavelengths, W, aperture_index,
+                           coating_min_ior, grid_count, grid_length, direction, out, inters,
+                           inters_count);
+}
+
+/* Same trace from element parameters given in double: radius, ior, height,
+ * centre, coating wavelength (each [L]) and Sellmeier coefficients [L][6]. */
+int FN(rfo_trace_params)(const double *radius, const double *ior, const double *height,
+                         const double *center, const double *coating, const double *coefficients,
+                         int L, int disperse, const int *paths, int P, const int *wavelengths,
+                         int W, int aperture_index, double coating_min_ior, int grid_count,
+                         double grid_length, const double *direction, RAY *out) {
+    ELEM el[256];
+    if (L > 256) return -1;
+    for (int i = 0; i < L; i++) {
+        el[i].radius = (REAL)radius[i];
+        el[i].ior = (REAL)ior[i];
+        el[i].height = (REAL)height[i];
+        el[i].center = (REAL)center[i];
+        el[i].coating = (REAL)coating[i];
+        for (int k = 0; k < 6; k++) el[i].coefficients[k] = (REAL)coefficients[i * 6 + k];
+    }
+    return FN(trace_elems)(el, L, disperse, paths, P, wavelengths, W, aperture_index,
+                           coating_min_ior, grid_count, grid_length, direction, out, NULL, 0);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -899,7 +948,9 @@ double FN(rfo_fresnel_ar)(double theta0, double lambda, double d1, double n0, do
                                   (REAL)n2);
 }
 double FN(rfo_dispersion)(double lambda, const float *coefficients) {
-    return (double)FN(dispersion)((REAL)lambda, coefficients);
+    REAL c[6];
+    for (int k = 0; k < 6; k++) c[k] = (REAL)coefficients[k];
+    return (double)FN(dispersion)((REAL)lambda, c);
 }
 int FN(rfo_intersect_quad)(int px, int py, const int *x, const int *y) {
     return FN(intersect_quad)(px, py, x, y);
@@ -913,6 +964,7 @@ double FN(rfo_sample_bilinear)(const float *tex, int w, int h, double s, double 
     return (double)FN(sample_bilinear)(tex, w, h, 1, (REAL)s, (REAL)t, edge);
 }
 
+#undef ELEM
 #undef V3
 #undef V2
 #undef RAY

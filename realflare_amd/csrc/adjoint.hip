@@ -1,0 +1,873 @@
+// Hand-written adjoint (reverse mode) of the ghost ray trace on MI355X.
+//
+// Forward semantics are those of the `raytrace` kernel
+// (realflare/api/tasks/cl/raytracing.cl:218-359, as in trace.hip): the same
+// per-event operations in the same order, so a ray's trajectory here is
+// bit-identical to rf_trace_planes_coated's.  Differentiated outputs per ray:
+// sensor position (x, y) and reflectance; parameters: element radius,
+// element centre (the host maps thicknesses onto centres), the coating
+// wavelength of every element, and the effective refractive index of every
+// element at every wavelength (the host chains Sellmeier B through it).
+//
+// One lane per ray, a one-wave workgroup per (ghost, wavelength, 64 rays), so
+// the surface sequence is wave-uniform.  The forward pass stores the ray state
+// every K events (checkpoints) in a per-ray scratch laid out [slot][field][ray]
+// (coalesced); the reverse pass walks the segments backwards, recomputes
+// each segment's per-event states from its checkpoint, then for every event
+// recomputes the event's intermediates (intersection, normal, Fresnel terms)
+// and applies their vector-Jacobian products.  Each lane accumulates its
+// parameter gradients in its own LDS column; the workgroup sums the columns
+// with a fixed butterfly into partials laid out [parameter][workgroup], which a
+// second kernel reduces per parameter in a fixed order: results are
+// deterministic and independent of scheduling.
+#include <cmath>
+
+#include "primitives.h"
+
+namespace rf {
+
+static constexpr int ADJ_THREADS = 64;  // one wave per workgroup
+static constexpr int ADJ_FIELDS = 7;  // pos.xyz, dir.xyz, reflectance
+
+struct St {
+    f3 pos, dir;
+    float refl;
+};
+
+// --- forward pieces (same arithmetic as trace.hip) ---------------------------
+
+__device__ __noinline__ float fresnel_f(float theta0, float n1, float n2) {
+#pragma clang fp contract(off)
+    float theta1 = asinf(sinf(theta0) * n1 / n2);
+    float cosi = cosf(theta0);
+    float cost = cosf(theta1);
+    float rs = ((n1 * cosi) - (n2 * cost)) / ((n1 * cosi) + (n2 * cost));
+    float rp = ((n1 * cost) - (n2 * cosi)) / ((n1 * cost) + (n2 * cosi));
+    return (rs * rs + rp * rp) / 2;
+}
+
+__device__ __noinline__ float fresnel_ar_f(float theta0, float lambda, float d1, float n0, float n1, float n2) {
+#pragma clang fp contract(off)
+    const float pi = 3.14159265358979323846f;
+    float s0 = sinf(theta0);
+    float theta1 = asinf(s0 * n0 / n1);
+    float theta2 = asinf(s0 * n0 / n2);
+    float sp01 = sinf(theta0 + theta1);
+    float rs01 = -sinf(theta0 - theta1) / sp01;
+    float rp01 = tanf(theta0 - theta1) / tanf(theta0 + theta1);
+    float ts01 = 2 * sinf(theta1) * cosf(theta0) / sp01;
+    float tp01 = ts01 * cosf(theta0 - theta1);
+    float rs12 = -sinf(theta1 - theta2) / sinf(theta1 + theta2);
+    float rp12 = tanf(theta1 - theta2) / tanf(theta1 + theta2);
+    float ris = ts01 * ts01 * rs12;
+    float rip = tp01 * tp01 * rp12;
+    float dy = d1 * n1;
+    float dx = tanf(theta1) * dy;
+    float delay = sqrtf(dx * dx + dy * dy);
+    float rel_phase = 4 * pi / lambda * (delay - dx * s0);
+    float c = cosf(rel_phase);
+    float out_s2 = rs01 * rs01 + ris * ris + 2 * rs01 * ris * c;
+    float out_p2 = rp01 * rp01 + rip * rip + 2 * rp01 * rip * c;
+    return (out_s2 + out_p2) / 2;
+}
+
+__device__ float dispersion_f(float lambda, const float* c) {
+#pragma clang fp contract(off)
+    float l = (float)((double)lambda * 1e-3);
+    float l2 = l * l;
+    float d0 = (c[0] * l2) / (l2 - c[1]);
+    float d1 = (c[2] * l2) / (l2 - c[3]);
+    float d2 = (c[4] * l2) / (l2 - c[5]);
+    return sqrtf(1 + d0 + d1 + d2);
+}
+
+struct Inter {
+    f3 p, n;
+    float incident;
+    bool hit;
+};
+
+__device__ __forceinline__ Inter intersect_fwd(f3 pos, f3 dir, float radius, float center,
+                                               bool want_incident) {
+#pragma clang fp contract(off)
+    Inter I;
+    I.p = mk3(0, 0, 0);
+    I.n = mk3(0, 0, 0);
+    I.incident = 0.0f;
+    if (dir.z == 0) {
+        I.hit = false;
+        return I;
+    }
+    if (radius == 0) {
+        const float dz = -center - pos.z;
+        I.p = add3(pos, scl3(dir, dz / dir.z));
+        I.n = (dir.z < 0) ? mk3(0, 0, 1) : mk3(0, 0, -1);
+        I.hit = true;
+        return I;
+    }
+    const float r = fabsf(radius);
+    const f3 C = mk3(0, 0, -center);
+    const f3 U = sub3(C, pos);
+    const f3 U1 = scl3(dir, dot3(U, dir));
+    const float d = len3(sub3(U, U1));
+    if (d > r) {
+        I.hit = false;
+        return I;
+    }
+    const float sgn = (radius * dir.z) > 0 ? -1.0f : 1.0f;
+    const float m = sqrtf(r * r - d * d);
+    I.p = sub3(add3(pos, U1), scl3(scl3(dir, m), sgn));
+    const f3 w = sub3(I.p, C);
+    const float l = len3(w);
+    I.n = scl3(mk3(w.x / l, w.y / l, w.z / l), sgn);
+    if (want_incident) I.incident = acosf(clampn(dot3(scl3(dir, -1.0f), I.n), -1.0f, 1.0f));
+    I.hit = true;
+    return I;
+}
+
+// --- reverse pieces -----------------------------------------------------------
+//
+// The reverse pass evaluates derivatives with the hardware reciprocal, square
+// root and reciprocal square root (about 1 ulp): a gradient needs no correctly
+// rounded intermediates, and no trajectory is ever advanced with them.
+
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
+
+// Surface geometry at the hit point p of a ray travelling along dir: the
+// normal as intersect() orients it and, for spheres, w = p - centre, 1/|w|.
+struct Hit {
+    f3 n, nh, w;
+    float il, sgn;
+    bool sphere;
+};
+
+__device__ __forceinline__ Hit hit_at(f3 dir, f3 p, float radius, float center) {
+    Hit h;
+    h.sphere = radius != 0;
+    if (!h.sphere) {
+        h.n = (dir.z < 0) ? mk3(0, 0, 1) : mk3(0, 0, -1);
+        return h;
+    }
+    h.w = sub3(p, mk3(0, 0, -center));
+    h.il = frsq(dot3(h.w, h.w));
+    h.nh = scl3(h.w, h.il);
+    h.sgn = (radius * dir.z) > 0 ? -1.0f : 1.0f;
+    h.n = scl3(h.nh, h.sgn);
+    return h;
+}
+
+// Adjoint of the intersection (raytracing.cl:104-168): hit point p, normal n
+// and, on spheres, the incident angle acos(-dir . n), given their adjoints pb,
+// nb, ib.  The sphere hit is differentiated implicitly: p = pos + t dir with
+// |p - C| = |radius| gives dt = (|radius| d|radius| - w . (dpos + t ddir - dC)) / (w . dir).
+__device__ __forceinline__ void hit_bwd(const Hit& h, f3 pos, f3 dir, f3 p, float radius,
+                                        float center, f3 pb, f3 nb, float ib, f3& posb, f3& dirb,
+                                        float& Rb, float& Cb) {
+#pragma clang fp contract(off)
+    if (!h.sphere) {
+        // p = pos + dir tau, tau = (-center - pos.z) / dir.z
+        const float idz = frcp(dir.z);
+        const float tau = (-center - pos.z) * idz;
+        posb = add3(posb, pb);
+        dirb = add3(dirb, scl3(pb, tau));
+        const float dzb = dot3(pb, dir) * idz;
+        dirb.z += -dzb * tau;
+        Cb += -dzb;
+        posb.z += -dzb;
+        return;
+    }
+    if (ib != 0.0f) {
+        const float c = -dot3(dir, h.n);
+        const float q = 1.0f - c * c;
+        if (c > -1.0f && c < 1.0f && q > 1e-12f) {
+            const float cb = -ib * frsq(q);
+            dirb = add3(dirb, scl3(h.n, -cb));
+            nb = add3(nb, scl3(dir, -cb));
+        }
+    }
+    // n = sgn w / |w|
+    const f3 wb = scl3(sub3(nb, scl3(h.nh, dot3(nb, h.nh))), h.sgn * h.il);
+    // w = p - C, C = (0, 0, -center)
+    pb = add3(pb, wb);
+    Cb += wb.z;
+    // p = pos + t dir
+    const float t = dot3(sub3(p, pos), dir);
+    const float lam = dot3(pb, dir) * frcp(dot3(h.w, dir));
+    const f3 q = sub3(pb, scl3(h.w, lam));
+    posb = add3(posb, q);
+    dirb = add3(dirb, scl3(q, t));
+    Cb += -lam * h.w.z;
+    Rb += lam * radius;
+}
+
+// refract (raytracing.cl:70-86) for a ray that did not totally reflect
+__device__ __forceinline__ void refract_bwd(f3 i, f3 n, float o, f3 tb, f3& ib, f3& nb, float& ob) {
+    const float cost = -dot3(i, n);
+    const float sint2 = (o * o) * (1 - cost * cost);
+    const float s = fsqrt(fabsf(1 - sint2));
+    const float k = o * cost - s;
+    ib = add3(ib, scl3(tb, o));
+    nb = add3(nb, scl3(tb, k));
+    const float kb = dot3(tb, n);
+    ob += dot3(tb, i) + kb * cost;
+    float costb = kb * o;
+    const float sb = -kb;
+    if (s > 0) {
+        const float qb = sb * 0.5f * frcp(s);
+        const float sint2b = (1 - sint2) >= 0 ? -qb : qb;
+        ob += sint2b * 2 * o * (1 - cost * cost);
+        costb += sint2b * o * o * (-2 * cost);
+    }
+    ib = add3(ib, scl3(n, -costb));
+    nb = add3(nb, scl3(i, -costb));
+}
+
+// reflect (raytracing.cl:96-102): r = i - 2 (n.i) n
+__device__ __forceinline__ void reflect_bwd(f3 i, f3 n, f3 rb, f3& ib, f3& nb) {
+    const float nr = dot3(n, rb), ni = dot3(n, i);
+    ib = add3(ib, sub3(rb, scl3(n, 2 * nr)));
+    nb = add3(nb, add3(scl3(i, -2 * nr), scl3(rb, -2 * ni)));
+}
+
+// fresnel (raytracing.cl:1-16): returns d/d(theta0, n1, n2) * Rb
+__device__ __forceinline__ void fresnel_bwd(float theta0, float n1, float n2, float Rb, float& t0b, float& n1b,
+                            float& n2b) {
+    const float s0 = sinf(theta0), c0 = cosf(theta0);
+    const float a1 = s0 * n1 / n2;
+    const float th1 = asinf(a1);
+    const float c1 = cosf(th1);
+    const float A = n1 * c0, B = n2 * c1, Cc = n1 * c1, D = n2 * c0;
+    const float rs = (A - B) / (A + B), rp = (Cc - D) / (Cc + D);
+    const float rsb = Rb * rs, rpb = Rb * rp;
+    // rs = (A - B)/(A + B): d/dA = 2B/(A+B)^2, d/dB = -2A/(A+B)^2
+    const float Ab = rsb * 2 * B / ((A + B) * (A + B));
+    const float Bb = -rsb * 2 * A / ((A + B) * (A + B));
+    const float Cb2 = rpb * 2 * D / ((Cc + D) * (Cc + D));
+    const float Db = -rpb * 2 * Cc / ((Cc + D) * (Cc + D));
+    n1b += Ab * c0 + Cb2 * c1;
+    n2b += Bb * c1 + Db * c0;
+    float c0b = Ab * n1 + Db * n2;
+    const float c1b = Bb * n2 + Cb2 * n1;
+    const float th1b = -c1b * sinf(th1);
+    float s0b = 0.0f;
+    const float q = 1 - a1 * a1;
+    if (q > 1e-12f) {
+        const float a1b = th1b / sqrtf(q);
+        s0b += a1b * n1 / n2;
+        n1b += a1b * s0 / n2;
+        n2b += -a1b * a1 / n2;
+    }
+    t0b += s0b * c0 - c0b * s0;
+}
+
+// fresnel_ar (raytracing.cl:18-59): gradient of R w.r.t. theta0, d1, n0, n1, n2
+__device__ __forceinline__ void fresnel_ar_bwd(float theta0, float lambda, float d1, float n0, float n1, float n2,
+                               float Rb, float& t0b, float& d1b, float& n0b, float& n1b,
+                               float& n2b) {
+    const float pi = 3.14159265358979323846f;
+    const float s0 = sinf(theta0), c0 = cosf(theta0);
+    const float a1 = s0 * n0 / n1, a2 = s0 * n0 / n2;
+    const float th1 = asinf(a1), th2 = asinf(a2);
+    const float dm = theta0 - th1, dp = theta0 + th1;
+    const float sdm = sinf(dm), sdp = sinf(dp), tdm = tanf(dm), tdp = tanf(dp);
+    const float rs01 = -sdm / sdp, rp01 = tdm / tdp;
+    const float s1 = sinf(th1);
+    const float ts01 = 2 * s1 * c0 / sdp;
+    const float cdm = cosf(dm);
+    const float tp01 = ts01 * cdm;
+    const float em = th1 - th2, ep = th1 + th2;
+    const float sem = sinf(em), sep = sinf(ep), tem = tanf(em), tep = tanf(ep);
+    const float rs12 = -sem / sep, rp12 = tem / tep;
+    const float ris = ts01 * ts01 * rs12, rip = tp01 * tp01 * rp12;
+    const float dy = d1 * n1;
+    const float t1 = tanf(th1);
+    const float dx = t1 * dy;
+    const float delay = sqrtf(dx * dx + dy * dy);
+    const float k4 = 4 * pi / lambda;
+    const float ph = k4 * (delay - dx * s0);
+    const float cph = cosf(ph);
+
+    const float osb = Rb * 0.5f, opb = Rb * 0.5f;
+    const float rs01b = osb * (2 * rs01 + 2 * ris * cph);
+    const float risb = osb * (2 * ris + 2 * rs01 * cph);
+    const float rp01b = opb * (2 * rp01 + 2 * rip * cph);
+    const float ripb = opb * (2 * rip + 2 * rp01 * cph);
+    const float cphb = osb * 2 * rs01 * ris + opb * 2 * rp01 * rip;
+    const float phb = -cphb * sinf(ph);
+    const float delayb = phb * k4;
+    float dxb = -phb * k4 * s0;
+    float s0b = -phb * k4 * dx;
+    float dyb = 0.0f;
+    if (delay > 0) {
+        dxb += delayb * dx / delay;
+        dyb += delayb * dy / delay;
+    }
+    const float t1b = dxb * dy;
+    dyb += dxb * t1;
+    d1b += dyb * n1;
+    n1b += dyb * d1;
+    float th1b = t1b * (1 + t1 * t1);
+    const float tp01b = ripb * 2 * tp01 * rp12;
+    const float rp12b = ripb * tp01 * tp01;
+    float ts01b = risb * 2 * ts01 * rs12;
+    const float rs12b = risb * ts01 * ts01;
+    const float temb = rp12b / tep, tepb = -rp12b * tem / (tep * tep);
+    const float semb = -rs12b / sep, sepb = rs12b * sem / (sep * sep);
+    const float emb = temb * (1 + tem * tem) + semb * cosf(em);
+    const float epb = tepb * (1 + tep * tep) + sepb * cosf(ep);
+    th1b += emb + epb;
+    float th2b = -emb + epb;
+    ts01b += tp01b * cdm;
+    const float cdmb = tp01b * ts01;
+    float dmb = -cdmb * sinf(dm);
+    const float s1b = ts01b * 2 * c0 / sdp;
+    float c0b = ts01b * 2 * s1 / sdp;
+    float sdpb = -ts01b * ts01 / sdp;
+    th1b += s1b * cosf(th1);
+    const float tdmb = rp01b / tdp, tdpb = -rp01b * tdm / (tdp * tdp);
+    const float sdmb = -rs01b / sdp;
+    sdpb += rs01b * sdm / (sdp * sdp);
+    dmb += tdmb * (1 + tdm * tdm) + sdmb * cosf(dm);
+    const float dpb = tdpb * (1 + tdp * tdp) + sdpb * cosf(dp);
+    float t0 = dmb + dpb;
+    th1b += -dmb + dpb;
+    const float q2 = 1 - a2 * a2;
+    if (q2 > 1e-12f) {
+        const float a2b = th2b / sqrtf(q2);
+        s0b += a2b * n0 / n2;
+        n0b += a2b * s0 / n2;
+        n2b += -a2b * a2 / n2;
+    }
+    const float q1 = 1 - a1 * a1;
+    if (q1 > 1e-12f) {
+        const float a1b = th1b / sqrtf(q1);
+        s0b += a1b * n0 / n1;
+        n0b += a1b * s0 / n1;
+        n1b += -a1b * a1 / n1;
+    }
+    t0 += s0b * c0 - c0b * s0;
+    t0b += t0;
+}
+
+// The Fresnel adjoints run on the two reflection events of a ghost only: kept
+// out of line, returning their partials by value.
+struct FresnelGrad {
+    float t0, d1, n0, n1, n2;
+};
+__device__ __noinline__ FresnelGrad fresnel_grad(float theta0, float n1, float n2, float Rb) {
+    FresnelGrad g{0, 0, 0, 0, 0};
+    fresnel_bwd(theta0, n1, n2, Rb, g.t0, g.n1, g.n2);
+    return g;
+}
+__device__ __noinline__ FresnelGrad fresnel_ar_grad(float theta0, float lambda, float d1, float n0,
+                                                    float n1, float n2, float Rb) {
+    FresnelGrad g{0, 0, 0, 0, 0};
+    fresnel_ar_bwd(theta0, lambda, d1, n0, n1, n2, Rb, g.t0, g.d1, g.n0, g.n1, g.n2);
+    return g;
+}
+
+// --- event bookkeeping ------------------------------------------------------------
+
+struct Path {
+    int b1, b2, E, L, stop;
+};
+
+__device__ __forceinline__ int event_element(const Path& P, int k) {
+    if (P.b1 < 0) return k;
+    if (k <= P.b1) return k;
+    if (k <= 2 * P.b1 - P.b2) return 2 * P.b1 - k;
+    return k - 2 * (P.b1 - P.b2);
+}
+__device__ __forceinline__ bool event_reflects(const Path& P, int k) {
+    return P.b1 >= 0 && (k == P.b1 || k == 2 * P.b1 - P.b2);
+}
+
+struct LensF {
+    const rf_lens_element* lens;
+    const float* coating;  // per element, or NULL
+    float cmin;
+    float wavelength;
+};
+
+// One event forward (trace.hip semantics).  Returns false when the ray dies.
+__device__ __forceinline__ bool event_fwd(const LensF& Lf, const float* s_n, int L, int stop, int el,
+                                          bool reflect, St& s) {
+#pragma clang fp contract(off)
+    const rf_lens_element& E = Lf.lens[el];
+    Inter I = intersect_fwd(s.pos, s.dir, E.radius, E.center, reflect);
+    if (!I.hit) return false;
+    const f3 dir_in = s.dir;
+    s.pos = I.p;
+    if (el == stop || el == L - 1) return true;
+    const int n_index = (dir_in.z < 0) ? el - 1 : el + 1;
+    const float n1 = (0 <= n_index && n_index < L) ? s_n[n_index] : 1.0f;
+    const float n2 = s_n[el];
+    if (reflect) {
+        s.dir = add3(dir_in, scl3(I.n, 2.0f * dot3(I.n, scl3(dir_in, -1.0f))));
+        const float theta = (float)((double)I.incident + 1e-9);
+        const float cf = Lf.coating ? Lf.coating[el] : (float)E.coating;
+        float r;
+        if (cf > 0) {
+            const float nc = fmaxf(sqrtf(n1 * n2), Lf.cmin);
+            const float d1 = cf / (4.0f * nc);
+            r = fresnel_ar_f(theta, Lf.wavelength, d1, n1, nc, n2);
+        } else {
+            r = fresnel_f(theta, n1, n2);
+        }
+        if (r > 0) s.refl *= r;
+        return true;
+    }
+    const float o = n1 / n2;
+    const float cost = dot3(scl3(dir_in, -1.0f), I.n);
+    const float sint2 = (o * o) * (1 - cost * cost);
+    f3 t = add3(scl3(dir_in, o), scl3(I.n, o * cost - sqrtf(fabsf(1 - sint2))));
+    s.dir = scl3(t, (float)(sint2 < 1));
+    return s.dir.z != 0;
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+// per-lane accumulation of one parameter's gradient: column `lane` of the
+// workgroup's [parameter][lane] array, summed over lanes once at the end
+__device__ __forceinline__ void acc(float* s_lacc, int idx, float v) {
+    s_lacc[idx * ADJ_THREADS + threadIdx.x] += v;
+}
+
+__device__ __forceinline__ void st_store(float* scratch, long N, long ray, int slot, const St& s) {
+    float* b = scratch + (long)slot * ADJ_FIELDS * N;
+    b[0 * N + ray] = s.pos.x;
+    b[1 * N + ray] = s.pos.y;
+    b[2 * N + ray] = s.pos.z;
+    b[3 * N + ray] = s.dir.x;
+    b[4 * N + ray] = s.dir.y;
+    b[5 * N + ray] = s.dir.z;
+    b[6 * N + ray] = s.refl;
+}
+__device__ __forceinline__ St st_load(const float* scratch, long N, long ray, int slot) {
+    const float* b = scratch + (long)slot * ADJ_FIELDS * N;
+    St s;
+    s.pos = mk3(b[0 * N + ray], b[1 * N + ray], b[2 * N + ray]);
+    s.dir = mk3(b[3 * N + ray], b[4 * N + ray], b[5 * N + ray]);
+    s.refl = b[6 * N + ray];
+    return s;
+}
+
+// Reverse of event k of a ray that reaches the sensor: `in` is the ray state
+// before the event and p its hit point (the position after it); (posb, dirb,
+// reflb) the adjoint of the state after the event on entry and before it on
+// return.  Parameter adjoints go to the lane's LDS column.
+__device__ __forceinline__ void event_rev(const LensF& Lf, const float* s_n, float* s_lacc,
+                                          const Path& P, int k, const St& in, f3 p, f3& posb,
+                                          f3& dirb, float& reflb) {
+#pragma clang fp contract(off)
+    const int L = P.L;
+    const int el = event_element(P, k);
+    const bool reflect = event_reflects(P, k);
+    const rf_lens_element& Ei = Lf.lens[el];
+    const Hit h = hit_at(in.dir, p, Ei.radius, Ei.center);
+    float Rb = 0.0f, Cb = 0.0f;
+    f3 nb = mk3(0, 0, 0), dirb_in = mk3(0, 0, 0);
+    float ib = 0.0f;
+    if (el == P.stop || el == L - 1) {
+        dirb_in = dirb;
+    } else {
+        const int n_index = (in.dir.z < 0) ? el - 1 : el + 1;
+        const bool inrange = 0 <= n_index && n_index < L;
+        const float n1 = inrange ? s_n[n_index] : 1.0f;
+        const float n2 = s_n[el];
+        float n1b = 0.0f, n2b = 0.0f;
+        if (reflect) {
+            reflect_bwd(in.dir, h.n, dirb, dirb_in, nb);
+            const float incident =
+                h.sphere ? acosf(clampn(-dot3(in.dir, h.n), -1.0f, 1.0f)) : 0.0f;
+            const float theta = (float)((double)incident + 1e-9);
+            const float cf = Lf.coating ? Lf.coating[el] : (float)Ei.coating;
+            float r, t0b = 0.0f;
+            if (cf > 0) {
+                const float q = sqrtf(n1 * n2);
+                const float nc = fmaxf(q, Lf.cmin);
+                const float d1 = cf / (4.0f * nc);
+                r = fresnel_ar_f(theta, Lf.wavelength, d1, n1, nc, n2);
+                if (r > 0) {
+                    const FresnelGrad fg =
+                        fresnel_ar_grad(theta, Lf.wavelength, d1, n1, nc, n2, reflb * in.refl);
+                    t0b = fg.t0;
+                    n1b = fg.n0;
+                    n2b = fg.n2;
+                    const float d1b = fg.d1;
+                    float ncb = fg.n1;
+                    acc(s_lacc, 2 * L + el, d1b / (4.0f * nc));
+                    ncb += -d1b * d1 / nc;
+                    if (q > Lf.cmin && q > 0) {
+                        n1b += ncb * n2 / (2 * q);
+                        n2b += ncb * n1 / (2 * q);
+                    }
+                }
+            } else {
+                r = fresnel_f(theta, n1, n2);
+                if (r > 0) {
+                    const FresnelGrad fg = fresnel_grad(theta, n1, n2, reflb * in.refl);
+                    t0b = fg.t0;
+                    n1b = fg.n1;
+                    n2b = fg.n2;
+                }
+            }
+            if (r > 0) reflb = reflb * r;
+            ib = t0b;
+        } else {
+            float ob = 0.0f;
+            const float in2 = frcp(n2);
+            refract_bwd(in.dir, h.n, n1 * in2, dirb, dirb_in, nb, ob);
+            n1b += ob * in2;
+            n2b += -ob * n1 * in2 * in2;
+        }
+        acc(s_lacc, 3 * L + el, n2b);
+        if (inrange) acc(s_lacc, 3 * L + n_index, n1b);
+    }
+    f3 posb_in = mk3(0, 0, 0);
+    hit_bwd(h, in.pos, in.dir, p, Ei.radius, Ei.center, posb, nb, ib, posb_in, dirb_in, Rb, Cb);
+    posb = posb_in;
+    dirb = dirb_in;
+    acc(s_lacc, el, Rb);
+    acc(s_lacc, L + el, Cb);
+}
+
+// What one launch of k_trace_adjoint does.
+enum AdjMode {
+    ADJ_FUSED = 0,    // forward, then reverse (the last segment's states stay in registers)
+    ADJ_FORWARD = 1,  // forward only: outputs, every segment's checkpoint and the final state
+    ADJ_REVERSE = 2,  // reverse only, from what an ADJ_FORWARD launch left in the scratch
+};
+
+// Partials (floats) are laid out [parameter][workgroup], parameters ordered
+// [radius L][centre L][coating L][n_eff L] (n_eff for the workgroup's wavelength).
+//
+// K events form a segment.  The forward pass keeps the current segment's input
+// states in registers (st[], indexed by unrolled loops) and stores segment
+// start states in the scratch (slot = segment; the final state, reflectance
+// NaN for a ray that does not reach the sensor, in slot nseg); the reverse
+// pass rebuilds each segment from its checkpoint, walking segments backwards.
+template <int K, int MODE>
+__global__ __launch_bounds__(ADJ_THREADS) void k_trace_adjoint(
+    const rf_lens_element* __restrict__ lens, const float* __restrict__ coating,
+    const int* __restrict__ paths, const int* __restrict__ wavelengths, rf_trace_params prm,
+    const float* __restrict__ gx, const float* __restrict__ gy, const float* __restrict__ gr,
+    float* __restrict__ scratch, float* __restrict__ partial, float* __restrict__ fwd_out) {
+#pragma clang fp contract(off)
+    __shared__ float s_n[RF_LMAX];
+    extern __shared__ float s_lacc[];  // [4 L][ADJ_THREADS]
+    const int L = prm.lens_count;
+    const int w = blockIdx.y, p = blockIdx.z;
+    const int g = prm.grid_count, R = g * g, W = prm.wavelength_count;
+    const float wavelength = (float)wavelengths[w];
+    const bool disperse = !isnan(lens[0].coefficients[0]);
+    if ((int)threadIdx.x < L) {
+        const rf_lens_element& e = lens[threadIdx.x];
+        float n = e.ior;
+        if (disperse && n > 1) n = dispersion_f(wavelength, e.coefficients);
+        s_n[threadIdx.x] = n;
+    }
+    if (MODE != ADJ_FORWARD)
+        for (int i = 0; i < 4 * L; i++) s_lacc[i * ADJ_THREADS + threadIdx.x] = 0.0f;
+    __syncthreads();
+
+    const long N = (long)prm.path_count * W * R;
+    const int ray_id = blockIdx.x * ADJ_THREADS + threadIdx.x;
+    const bool active = ray_id < R;
+    const long idx = ((long)p * W + w) * R + (active ? ray_id : 0);
+    LensF Lf{lens, coating, prm.coating_min_ior, wavelength};
+    Path P;
+    P.b1 = paths[2 * p];
+    P.b2 = paths[2 * p + 1];
+    P.L = L;
+    P.stop = prm.aperture_index;
+    P.E = P.b1 >= 0 ? L + 2 * (P.b1 - P.b2) : L;
+    const int nseg = (P.E + K - 1) / K;
+
+    // init_ray (raytracing.cl:178-215)
+    const int ray_y = (int)truncf((float)(active ? ray_id : 0) / (float)g);
+    const int ray_x = (active ? ray_id : 0) - ray_y * g;
+    const f3 gpt = mk3((float)((double)prm.grid_length * ((double)((float)ray_x / (float)(g - 1)) - 0.5)),
+                       (float)((double)prm.grid_length * (0.5 - (double)((float)ray_y / (float)(g - 1)))),
+                       1.0f);
+    const f3 down = mk3(0, 0, -1.0f);
+    const Inter I0 = intersect_fwd(gpt, down, lens[0].radius, lens[0].center, false);
+    const f3 d0 = norm3(mk3(-prm.direction[0], -prm.direction[1], -prm.direction[2]));
+    St s;
+    s.pos = sub3(I0.p, d0);
+    s.dir = d0;
+    s.refl = 1.0f;
+
+    St st[K];
+    bool alive = active;
+    if (MODE != ADJ_REVERSE) {
+        for (int seg = 0; seg < nseg; seg++) {
+            const int k0 = seg * K;
+            if (active && (MODE == ADJ_FORWARD || seg + 1 < nseg)) st_store(scratch, N, idx, seg, s);
+#pragma unroll
+            for (int j = 0; j < K; j++) {
+                if (k0 + j < P.E) {
+                    st[j] = s;
+                    if (alive)
+                        alive = event_fwd(Lf, s_n, L, P.stop, event_element(P, k0 + j),
+                                          event_reflects(P, k0 + j), s);
+                }
+            }
+        }
+        if (fwd_out && active) {
+            fwd_out[idx] = alive ? s.pos.x : NAN;
+            fwd_out[N + idx] = alive ? s.pos.y : NAN;
+            fwd_out[2 * N + idx] = alive ? s.refl : NAN;
+        }
+        if (MODE == ADJ_FORWARD) {
+            if (active) {
+                St fin = s;
+                if (!alive) fin.refl = NAN;
+                st_store(scratch, N, idx, nseg, fin);
+            }
+            return;
+        }
+    } else if (active) {
+        s = st_load(scratch, N, idx, nseg);
+        alive = !isnan(s.refl);
+    }
+    // reverse (rays that do not reach the sensor contribute nothing)
+    if (alive) {
+        f3 posb = mk3(gx[idx], gy[idx], 0.0f), dirb = mk3(0, 0, 0);
+        float reflb = gr[idx];
+        f3 hit = s.pos;  // hit point of the event being reversed
+        for (int seg = nseg - 1; seg >= 0; seg--) {
+            const int k0 = seg * K;
+            if (MODE == ADJ_REVERSE || seg + 1 < nseg) {
+                st[0] = st_load(scratch, N, idx, seg);
+#pragma unroll
+                for (int j = 1; j < K; j++) {
+                    if (k0 + j < P.E) {
+                        st[j] = st[j - 1];
+                        event_fwd(Lf, s_n, L, P.stop, event_element(P, k0 + j - 1),
+                                  event_reflects(P, k0 + j - 1), st[j]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int j = K - 1; j >= 0; j--) {
+                if (k0 + j < P.E) {
+                    event_rev(Lf, s_n, s_lacc, P, k0 + j, st[j], hit, posb, dirb, reflb);
+                    hit = st[j].pos;
+                }
+            }
+        }
+        // init_ray: pos0 = I0.p - d0 with the vertical probe ray; d0 is constant
+        if (I0.hit) {
+            float Rb = 0.0f, Cb = 0.0f;
+            f3 gpb = mk3(0, 0, 0), ddb = mk3(0, 0, 0);
+            const Hit h0 = hit_at(down, I0.p, lens[0].radius, lens[0].center);
+            hit_bwd(h0, gpt, down, I0.p, lens[0].radius, lens[0].center, posb, mk3(0, 0, 0), 0.0f,
+                    gpb, ddb, Rb, Cb);
+            acc(s_lacc, 0, Rb);
+            acc(s_lacc, L, Cb);
+        }
+    }
+    const long blk = ((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+    const long nblk = (long)gridDim.x * gridDim.y * gridDim.z;
+    for (int i = 0; i < 4 * L; i++) {
+        const float v = wave_sum(s_lacc[i * ADJ_THREADS + threadIdx.x]);
+        if (threadIdx.x == 0) partial[i * nblk + blk] = v;
+    }
+}
+
+static constexpr int RED_THREADS = 256;
+
+// out[0:L) radius, [L:2L) centre, [2L:3L) coating, [3L + w L + i) n_eff[w][i].
+// One workgroup per output: lane j sums the output's workgroup partials
+// j, j + 256, ... in order, then the lanes combine in a fixed tree.
+__global__ __launch_bounds__(RED_THREADS) void k_adjoint_reduce(const float* __restrict__ partial,
+                                                                 int L, int bx, int W, int P,
+                                                                 float* __restrict__ out) {
+    __shared__ float s_w[RED_THREADS / 64];
+    const int t = blockIdx.x;
+    const long nblk = (long)bx * W * P;
+    float v = 0.0f;
+    if (t < 3 * L) {
+        const float* src = partial + (long)t * nblk;
+        for (long j = threadIdx.x; j < nblk; j += RED_THREADS) v += src[j];
+    } else {
+        const int w = (t - 3 * L) / L, i = (t - 3 * L) % L;
+        const float* src = partial + (long)(3 * L + i) * nblk;
+        const long n = (long)P * bx;  // the workgroups of wavelength w: bx per ghost
+        for (long j = threadIdx.x; j < n; j += RED_THREADS) {
+            const long p = j / bx, x = j - p * bx;
+            v += src[(p * W + w) * bx + x];
+        }
+    }
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float r = 0.0f;
+        for (int k = 0; k < RED_THREADS / 64; k++) r += s_w[k];
+        out[t] = r;
+    }
+}
+
+typedef void (*AdjKernel)(const rf_lens_element*, const float*, const int*, const int*,
+                          rf_trace_params, const float*, const float*, const float*, float*, float*,
+                          float*);
+
+template <int MODE>
+static AdjKernel adj_kernel_k(int K) {
+    switch (K) {
+        case 1: return k_trace_adjoint<1, MODE>;
+        case 2: return k_trace_adjoint<2, MODE>;
+        case 3: return k_trace_adjoint<3, MODE>;
+        case 4: return k_trace_adjoint<4, MODE>;
+        case 6: return k_trace_adjoint<6, MODE>;
+        case 8: return k_trace_adjoint<8, MODE>;
+        default: return nullptr;
+    }
+}
+
+// The segment length is a compile-time constant so that a segment's states
+// live in registers.
+static AdjKernel adj_kernel(int K, int mode = ADJ_FUSED) {
+    switch (mode) {
+        case ADJ_FUSED: return adj_kernel_k<ADJ_FUSED>(K);
+        case ADJ_FORWARD: return adj_kernel_k<ADJ_FORWARD>(K);
+        case ADJ_REVERSE: return adj_kernel_k<ADJ_REVERSE>(K);
+        default: return nullptr;
+    }
+}
+
+static int max_events(const int* host_paths, int P, int L) {
+    (void)host_paths;
+    (void)P;
+    return 3 * L;
+}
+
+struct AdjLayout {
+    size_t scratch, partial, total;
+    int slots, bx;
+};
+
+static AdjLayout adj_layout(const rf_trace_params* prm, int K) {
+    AdjLayout A;
+    const int L = prm->lens_count;
+    const long R = (long)prm->grid_count * prm->grid_count;
+    const long N = (long)prm->path_count * prm->wavelength_count * R;
+    const int E = max_events(nullptr, prm->path_count, L);
+    A.slots = (E + K - 1) / K + 1;  // segment starts and the final state
+    A.bx = (int)((R + ADJ_THREADS - 1) / ADJ_THREADS);
+    size_t o = 0;
+    A.scratch = o;
+    o += align_up((size_t)A.slots * ADJ_FIELDS * N * sizeof(float), 256);
+    A.partial = o;
+    o += align_up((size_t)A.bx * prm->wavelength_count * prm->path_count * 4 * L * sizeof(float), 256);
+    A.total = o;
+    return A;
+}
+
+}  // namespace rf
+
+using namespace rf;
+
+extern "C" size_t rf_trace_adjoint_workspace_size(const rf_trace_params* prm, int checkpoint_every) {
+    if (!prm || !adj_kernel(checkpoint_every) || prm->lens_count < 1 || prm->lens_count > RF_LMAX) return 0;
+    return adj_layout(prm, checkpoint_every).total;
+}
+
+// Argument checks shared by the three entry points; fills A.
+static int adj_check(const char* fn, const rf_lens_element* lens, const int32_t* paths,
+                     const int32_t* wavelengths, const rf_trace_params* prm, int K,
+                     const void* workspace, size_t workspace_bytes, AdjLayout& A) {
+    if (!prm || !lens || !paths || !wavelengths)
+        return set_error(RF_ERR_INVALID, "%s: NULL argument", fn);
+    if (prm->lens_count < 1 || prm->lens_count > RF_LMAX)
+        return set_error(RF_ERR_INVALID, "%s: lens_count %d", fn, prm->lens_count);
+    if (!adj_kernel(K))
+        return set_error(RF_ERR_INVALID, "%s: checkpoint_every %d not in {1, 2, 3, 4, 6, 8}", fn, K);
+    if (prm->path_count < 1 || prm->wavelength_count < 1 || prm->grid_count < 2)
+        return set_error(RF_ERR_INVALID, "%s: empty trace", fn);
+    A = adj_layout(prm, K);
+    if (!workspace || workspace_bytes < A.total)
+        return set_error(RF_ERR_WORKSPACE, "%s: workspace %zu < %zu", fn, workspace_bytes, A.total);
+    return RF_OK;
+}
+
+static int adj_launch(int mode, const rf_lens_element* lens, const float* coating_lambda,
+                      const int32_t* paths, const int32_t* wavelengths, const rf_trace_params* prm,
+                      int K, const float* gx, const float* gy, const float* gr, const AdjLayout& A,
+                      void* workspace, float* grad_out, float* forward_out, rf_stream_t stream) {
+    hipStream_t s = (hipStream_t)stream;
+    char* w = (char*)workspace;
+    dim3 grid(A.bx, prm->wavelength_count, prm->path_count);
+    const size_t lds =
+        mode == ADJ_FORWARD ? 0 : (size_t)4 * prm->lens_count * ADJ_THREADS * sizeof(float);
+    hipLaunchKernelGGL(adj_kernel(K, mode), grid, dim3(ADJ_THREADS), lds, s, lens, coating_lambda,
+                       paths, wavelengths, *prm, gx, gy, gr, (float*)(w + A.scratch),
+                       (float*)(w + A.partial), forward_out);
+    if (mode != ADJ_FORWARD) {
+        const int L = prm->lens_count, nout = 3 * L + prm->wavelength_count * L;
+        hipLaunchKernelGGL(k_adjoint_reduce, dim3(nout), dim3(RED_THREADS), 0, s,
+                           (const float*)(w + A.partial), L, A.bx, prm->wavelength_count,
+                           prm->path_count, grad_out);
+    }
+    RF_LAUNCH_CHECK("rf_trace_adjoint");
+    return RF_OK;
+}
+
+extern "C" int rf_trace_adjoint(const rf_lens_element* lens, const float* coating_lambda,
+                                const int32_t* paths, const int32_t* wavelengths,
+                                const rf_trace_params* prm, int checkpoint_every,
+                                const float* grad_pos_x, const float* grad_pos_y,
+                                const float* grad_reflectance, void* workspace,
+                                size_t workspace_bytes, float* grad_out, float* forward_out,
+                                rf_stream_t stream) {
+    AdjLayout A;
+    int rc = adj_check("rf_trace_adjoint", lens, paths, wavelengths, prm, checkpoint_every,
+                       workspace, workspace_bytes, A);
+    if (rc) return rc;
+    if (!grad_pos_x || !grad_pos_y || !grad_reflectance || !grad_out)
+        return set_error(RF_ERR_INVALID, "rf_trace_adjoint: NULL gradient argument");
+    return adj_launch(ADJ_FUSED, lens, coating_lambda, paths, wavelengths, prm, checkpoint_every,
+                      grad_pos_x, grad_pos_y, grad_reflectance, A, workspace, grad_out, forward_out,
+                      stream);
+}
+
+extern "C" int rf_trace_forward_checkpointed(const rf_lens_element* lens,
+                                             const float* coating_lambda, const int32_t* paths,
+                                             const int32_t* wavelengths,
+                                             const rf_trace_params* prm, int checkpoint_every,
+                                             void* workspace, size_t workspace_bytes,
+                                             float* forward_out, rf_stream_t stream) {
+    AdjLayout A;
+    int rc = adj_check("rf_trace_forward_checkpointed", lens, paths, wavelengths, prm,
+                       checkpoint_every, workspace, workspace_bytes, A);
+    if (rc) return rc;
+    if (!forward_out)
+        return set_error(RF_ERR_INVALID, "rf_trace_forward_checkpointed: NULL forward_out");
+    return adj_launch(ADJ_FORWARD, lens, coating_lambda, paths, wavelengths, prm, checkpoint_every,
+                      nullptr, nullptr, nullptr, A, workspace, nullptr, forward_out, stream);
+}
+
+extern "C" int rf_trace_backward(const rf_lens_element* lens, const float* coating_lambda,
+                                 const int32_t* paths, const int32_t* wavelengths,
+                                 const rf_trace_params* prm, int checkpoint_every,
+                                 const float* grad_pos_x, const float* grad_pos_y,
+                                 const float* grad_reflectance, void* workspace,
+                                 size_t workspace_bytes, float* grad_out, rf_stream_t stream) {
+    AdjLayout A;
+    int rc = adj_check("rf_trace_backward", lens, paths, wavelengths, prm, checkpoint_every,
+                       workspace, workspace_bytes, A);
+    if (rc) return rc;
+    if (!grad_pos_x || !grad_pos_y || !grad_reflectance || !grad_out)
+        return set_error(RF_ERR_INVALID, "rf_trace_backward: NULL gradient argument");
+    return adj_launch(ADJ_REVERSE, lens, coating_lambda, paths, wavelengths, prm, checkpoint_every,
+                      grad_pos_x, grad_pos_y, grad_reflectance, A, workspace, grad_out, nullptr,
+                      stream);
+}

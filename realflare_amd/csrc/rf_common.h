@@ -43,13 +43,31 @@ struct f3 {
     float x, y, z;
 };
 
+// The helpers below are compiled without FP contraction, like the kernels
+// that use them, so every kernel evaluates a shared expression identically.
 __device__ __forceinline__ f3 mk3(float x, float y, float z) { return f3{x, y, z}; }
-__device__ __forceinline__ f3 add3(f3 a, f3 b) { return mk3(a.x + b.x, a.y + b.y, a.z + b.z); }
-__device__ __forceinline__ f3 sub3(f3 a, f3 b) { return mk3(a.x - b.x, a.y - b.y, a.z - b.z); }
-__device__ __forceinline__ f3 scl3(f3 a, float s) { return mk3(a.x * s, a.y * s, a.z * s); }
-__device__ __forceinline__ float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ float len3(f3 a) { return sqrtf(dot3(a, a)); }
+__device__ __forceinline__ f3 add3(f3 a, f3 b) {
+#pragma clang fp contract(off)
+    return mk3(a.x + b.x, a.y + b.y, a.z + b.z);
+}
+__device__ __forceinline__ f3 sub3(f3 a, f3 b) {
+#pragma clang fp contract(off)
+    return mk3(a.x - b.x, a.y - b.y, a.z - b.z);
+}
+__device__ __forceinline__ f3 scl3(f3 a, float s) {
+#pragma clang fp contract(off)
+    return mk3(a.x * s, a.y * s, a.z * s);
+}
+__device__ __forceinline__ float dot3(f3 a, f3 b) {
+#pragma clang fp contract(off)
+    return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+__device__ __forceinline__ float len3(f3 a) {
+#pragma clang fp contract(off)
+    return sqrtf(dot3(a, a));
+}
 __device__ __forceinline__ f3 norm3(f3 a) {
+#pragma clang fp contract(off)
     float l = len3(a);
     return mk3(a.x / l, a.y / l, a.z / l);
 }
@@ -57,6 +75,7 @@ __device__ __forceinline__ f3 norm3(f3 a) {
 __device__ __forceinline__ float clampn(float x, float a, float b) { return fminf(fmaxf(x, a), b); }
 
 __device__ __forceinline__ float smoothstepf(float e0, float e1, float x) {
+#pragma clang fp contract(off)
     float t = clampn((x - e0) / (e1 - e0), 0.0f, 1.0f);
     return t * t * (3.0f - 2.0f * t);
 }
@@ -77,6 +96,7 @@ __device__ __forceinline__ int wadd(int a, int b) { return (int)((uint32_t)a + (
 // border 0; edge=true: CLK_ADDRESS_CLAMP_TO_EDGE.
 __device__ __forceinline__ float sample_bilinear(const float* __restrict__ tex, int w, int h,
                                                  int stride, float s, float t, bool edge) {
+#pragma clang fp contract(off)
     float u = s * (float)w - 0.5f, v = t * (float)h - 0.5f;
     float fu = floorf(u), fv = floorf(v);
     int i0 = rf_to_int(fu), j0 = rf_to_int(fv);
@@ -109,6 +129,7 @@ __device__ __forceinline__ float sample_bilinear(const float* __restrict__ tex, 
 // color.cl:3-10, with the matrix applied in double as the source's double
 // literals ask for.
 __device__ __forceinline__ void xyz_to_ap1(float x, float y, float z, float* o) {
+#pragma clang fp contract(off)
     double dx = x, dy = y, dz = z;
     o[0] = (float)(1.6410233797 * dx + -0.3248032942 * dy + -0.2364246952 * dz);
     o[1] = (float)(-0.6636628587 * dx + 1.6153315917 * dy + 0.0167563477 * dz);

@@ -113,7 +113,8 @@ __global__ __launch_bounds__(256) void k_trace(const rf_lens_element* __restrict
                                                const int* __restrict__ paths,
                                                const int* __restrict__ wavelengths,
                                                rf_trace_params prm, void* __restrict__ out,
-                                               void* __restrict__ inters) {
+                                               void* __restrict__ inters,
+                                               const float* __restrict__ coating_lambda) {
 #pragma clang fp contract(off)
     __shared__ float s_n[RF_LMAX];
     const int L = prm.lens_count;
@@ -161,6 +162,7 @@ __global__ __launch_bounds__(256) void k_trace(const rf_lens_element* __restrict
         const float center = lens[lens_id].center;
         const float height = lens[lens_id].height;
         const int coating = lens[lens_id].coating;
+        const float coating_f = coating_lambda ? coating_lambda[lens_id] : (float)coating;
         const bool is_stop = lens_id == aperture;
         const bool is_sensor = lens_id == L - 1;
         const bool do_reflect =
@@ -196,8 +198,8 @@ __global__ __launch_bounds__(256) void k_trace(const rf_lens_element* __restrict
                             dir = add3(dir, scl3(h.normal, 2.0f * dot3(h.normal, scl3(dir, -1.0f))));
                             float theta = (float)((double)h.incident + 1e-9);
                             float r;
-                            if (coating > 0) {
-                                float lambda_c = (float)coating;
+                            if (coating_f > 0) {
+                                float lambda_c = coating_f;
                                 float nc = fmaxf(sqrtf(n1 * n2), prm.coating_min_ior);
                                 float d1 = lambda_c / (4.0f * nc);
                                 r = fresnel_ar(theta, wavelength, d1, n1, nc, n2);
@@ -273,14 +275,14 @@ static int check_trace(const rf_lens_element* lens, const int32_t* paths,
 template <int OUT>
 static int launch_trace(const rf_lens_element* lens, const int32_t* paths,
                         const int32_t* wavelengths, const rf_trace_params* prm, void* out,
-                        void* inters, rf_stream_t stream) {
+                        void* inters, rf_stream_t stream, const float* coating_lambda = nullptr) {
     int rc = check_trace(lens, paths, wavelengths, prm, out);
     if (rc) return rc;
     if (prm->path_count == 0 || prm->wavelength_count == 0) return RF_OK;
     const int R = prm->grid_count * prm->grid_count;
     dim3 grid((R + 255) / 256, prm->wavelength_count, prm->path_count);
     hipLaunchKernelGGL(k_trace<OUT>, grid, dim3(256), 0, (hipStream_t)stream, lens, paths,
-                       wavelengths, *prm, out, inters);
+                       wavelengths, *prm, out, inters, coating_lambda);
     RF_LAUNCH_CHECK("k_trace");
     return RF_OK;
 }
@@ -306,4 +308,13 @@ extern "C" int rf_trace_intersections(const rf_lens_element* lens, const int32_t
     if (!intersections) return rf::set_error(RF_ERR_INVALID, "rf_trace_intersections: NULL");
     return rf::launch_trace<rf::OUT_INTERSECTIONS>(lens, paths, wavelengths, prm, rays,
                                                    intersections, stream);
+}
+
+extern "C" int rf_trace_planes_coated(const rf_lens_element* lens, const float* coating_lambda,
+                                      const int32_t* paths, const int32_t* wavelengths,
+                                      const rf_trace_params* prm, float* planes,
+                                      rf_stream_t stream) {
+    if (!coating_lambda) return rf::set_error(RF_ERR_INVALID, "rf_trace_planes_coated: NULL");
+    return rf::launch_trace<rf::OUT_PLANES>(lens, paths, wavelengths, prm, planes, nullptr, stream,
+                                            coating_lambda);
 }
