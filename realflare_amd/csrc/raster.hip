@@ -63,7 +63,11 @@ struct alignas(16) Attr {
     float x, y, z, pad_;
 };
 
-__constant__ unsigned char c_sub_offsets[15] = {0, 1, 0, 1, 2, 0, 3, 4, 1, 6, 2, 5, 0, 3, 7};
+// sub_offsets of rasterizing.cl (y offset of AA sample s at index AA - 1 + s:
+// 0 | 1, 0 | 1, 2, 0, 3 | 4, 1, 6, 2, 5, 0, 3, 7), 3 bits per entry, kept in a
+// register-resident immediate.
+static constexpr uint64_t SUB_OFFSETS = 0x1d8ab18c2208ull;
+__host__ __device__ __forceinline__ int sub_offset(int i) { return (int)((SUB_OFFSETS >> (3 * i)) & 7); }
 
 __device__ __forceinline__ long floordiv(long a, long b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
 __device__ __forceinline__ long ceildiv(long a, long b) { return -floordiv(-a, b); }
@@ -574,6 +578,70 @@ __device__ __forceinline__ void barycentric_fast(int px, int py, const int* x, c
     for (int i = 0; i < 4; i++) w[i] = ww[i] * inv;
 }
 
+// Workgroups are dealt to the XCDs round-robin (workgroup i runs on XCD
+// i % RF_XCDS) and each XCD has its own L2.  A quad's records are read by every
+// tile it overlaps, so neighbouring tiles should share an L2: the screen is cut
+// into SB x SB-tile super-blocks, super-block b goes to XCD b % RF_XCDS (which
+// keeps the XCDs balanced), and an XCD walks its super-blocks in order, the
+// SB * SB tiles of one super-block back to back.
+static constexpr int RF_XCDS = 8;
+static constexpr int SB = 4;
+
+__host__ __device__ __forceinline__ long raster_grid(const Geom& G) {
+    const long nsb = (long)((G.ntx + SB - 1) / SB) * ((G.nty + SB - 1) / SB);
+    const long per_xcd = (nsb + RF_XCDS - 1) / RF_XCDS;
+    return per_xcd * RF_XCDS * SB * SB;
+}
+
+__device__ __forceinline__ bool raster_tile(unsigned bid, const Geom& G, int* tx, int* ty) {
+    const unsigned xcd = bid % RF_XCDS, j = bid / RF_XCDS;
+    const unsigned sb = (j / (SB * SB)) * RF_XCDS + xcd, t = j % (SB * SB);
+    const int nsbx = (G.ntx + SB - 1) / SB;
+    *tx = (int)(sb % nsbx) * SB + (int)(t % SB);
+    *ty = (int)(sb / nsbx) * SB + (int)(t / SB);
+    return *tx < G.ntx && *ty < G.nty;
+}
+
+// fragment_shader (rasterizing.cl:471-506) for one covered pixel of one quad
+// step, scaled by its covered AA samples.
+__device__ __forceinline__ float shade_fragment(int pcx, int pcy, const Setup& rec, const Attr& at,
+                                                int hits, const float* __restrict__ ghost,
+                                                const Geom& G, float inv_scale) {
+#pragma clang fp contract(off)
+    float w[4];
+    barycentric_fast(pcx, pcy, rec.vx, rec.vy, w);
+    float uu = w[0] * at.u[0] + w[1] * at.u[1] + w[2] * at.u[2] + w[3] * at.u[3];
+    float vv = w[0] * at.v[0] + w[1] * at.v[1] + w[2] * at.v[2] + w[3] * at.v[3];
+    uu = (uu * inv_scale + 1.0f) * 0.5f;
+    vv = (vv * inv_scale + 1.0f) * 0.5f;
+    const float gi = sample_bilinear(ghost, G.gw, G.gh, 1, uu, vv, false);
+    const float rel = w[0] * at.rrel[0] + w[1] * at.rrel[1] + w[2] * at.rrel[2] + w[3] * at.rrel[3];
+    // smoothstep(1, 0.95, rrel)
+    const float tt = clampn(cl_div(rel - 1.0f, 0.95f - 1.0f), 0.0f, 1.0f);
+    const float ri = tt * tt * (3.0f - 2.0f * tt);
+    float ai = w[0] * at.inten[0] + w[1] * at.inten[1] + w[2] * at.inten[2] + w[3] * at.inten[3];
+    float ci = w[0] * at.refl[0] + w[1] * at.refl[1] + w[2] * at.refl[2] + w[3] * at.refl[3];
+    ai = fmaxf(ai, 0.0f);
+    ci = fmaxf(ci, 0.0f);
+    return 0.0f + gi * ri * ai * ci * (float)hits;
+}
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+}
+
+__device__ __forceinline__ float lane_value(float v, uint32_t src_lane) {
+    return __int_as_float(__builtin_amdgcn_ds_bpermute((int)(src_lane << 2), __float_as_int(v)));
+}
+
+// rasterizer (rasterizing.cl:524-663): one wave per 8x8 tile, lane = pixel.
+// The tile's (quad step) pairs are walked in binning order.  A pair's coverage
+// is tested by every pixel lane; the covered pixels are appended to a 64-entry
+// fragment queue in LDS (pixel lane, covered samples, pair slot).  When the
+// queue is full, lane i shades fragment i, and the colours go back to their
+// pixel lanes slot by slot, i.e. in pair order, so each pixel adds exactly the
+// terms of the per-pixel loop in the same order.  Shading thus runs on full
+// waves even when a quad covers few pixels of the tile.
 __global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ bounds,
                                                const float* __restrict__ ghost,
                                                const Setup* __restrict__ setup,
@@ -584,17 +652,58 @@ __global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ bounds
                                                const Header* __restrict__ h, Geom G,
                                                float4* __restrict__ image) {
 #pragma clang fp contract(off)
+    __shared__ uint32_t s_q[64];  // pixel lane | hits << 6 | slot << 10
     if (h->overflow) return;
-    const int tile = blockIdx.x;
-    const int tx = tile % G.ntx, ty = tile / G.ntx;
+    int tx, ty;
+    if (!raster_tile(blockIdx.x, G, &tx, &ty)) return;
+    const int tile = ty * G.ntx + tx;
     const int lane = threadIdx.x;
     const int x = tx * TILE + (lane & (TILE - 1)), y = ty * TILE + (lane / TILE);
     const bool inside = x < G.width && y < G.height;
     const int px = x * G.AA, py = y * G.AA;
-    const int pcx = px + G.AA / 2, pcy = py + G.AA / 2;
     const int bxp = x / G.bin, byp = y / G.bin;
     const float inv_scale = 1.0f / G.ghost_scale;
     float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f;
+    // queue state (wave-uniform) and per-slot records, slot s held by lane s
+    uint32_t qlen = 0, nslot = 0;
+    uint64_t slot_mask = 0;
+    uint32_t slot_lo = 0, slot_e = 0;
+
+    auto flush = [&]() {
+        __syncthreads();  // queue entries written by other lanes are visible
+        float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
+        if ((uint32_t)lane < qlen) {
+            const uint32_t ent = s_q[lane];
+            const int p = ent & 63, hits = (ent >> 6) & 15;
+            const uint32_t e = (uint32_t)__builtin_amdgcn_ds_bpermute((int)((ent >> 10) << 2),
+                                                                     (int)slot_e);
+            const Setup rec = setup[e];
+            const Attr at = attr[e];
+            const int fx = tx * TILE + (p & (TILE - 1)), fy = ty * TILE + p / TILE;
+            const float frag = shade_fragment(fx * G.AA + G.AA / 2, fy * G.AA + G.AA / 2, rec, at,
+                                              hits, ghost, G, inv_scale);
+            c0 = at.x * frag;
+            c1 = at.y * frag;
+            c2 = at.z * frag;
+        }
+        for (uint32_t sl = 0; sl < nslot; sl++) {
+            const uint32_t mlo = __builtin_amdgcn_readlane((uint32_t)slot_mask, sl);
+            const uint32_t mhi = __builtin_amdgcn_readlane((uint32_t)(slot_mask >> 32), sl);
+            const uint32_t lo = __builtin_amdgcn_readlane(slot_lo, sl);
+            const uint64_t m = ((uint64_t)mhi << 32) | mlo;
+            const uint32_t src = lo + lanes_below(m);
+            const float v0 = lane_value(c0, src), v1 = lane_value(c1, src), v2 = lane_value(c2, src);
+            if ((m >> lane) & 1) {
+                r0 += v0;
+                r1 += v1;
+                r2 += v2;
+            }
+        }
+        qlen = 0;
+        nslot = 0;
+        __syncthreads();  // every entry has been read before the queue is refilled
+    };
+
     const uint32_t start = tstart[tile], end = tend[tile];
     for (uint32_t k = start; k < end; k++) {
         const uint32_t e = vals[k];
@@ -612,38 +721,24 @@ __global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ bounds
         }
         if (ok) {
             for (int s = 0; s < G.AA; s++) {
-                const int sx = px + s, sy = py + c_sub_offsets[G.AA - 1 + s];
+                const int sx = px + s, sy = py + sub_offset(G.AA - 1 + s);
                 if (intersect_quad(sx, sy, rec.vx, rec.vy, rec.flags)) hits++;
             }
         }
-        if (!__any(hits > 0)) continue;
-        const Attr at = attr[e];
-        if (hits > 0) {
-            float w[4];
-            barycentric_fast(pcx, pcy, rec.vx, rec.vy, w);
-            // fragment_shader, rasterizing.cl:471-506
-            float uu = w[0] * at.u[0] + w[1] * at.u[1] + w[2] * at.u[2] + w[3] * at.u[3];
-            float vv = w[0] * at.v[0] + w[1] * at.v[1] + w[2] * at.v[2] + w[3] * at.v[3];
-            uu = (uu * inv_scale + 1.0f) * 0.5f;
-            vv = (vv * inv_scale + 1.0f) * 0.5f;
-            const float gi = sample_bilinear(ghost, G.gw, G.gh, 1, uu, vv, false);
-            const float rel = w[0] * at.rrel[0] + w[1] * at.rrel[1] + w[2] * at.rrel[2] +
-                              w[3] * at.rrel[3];
-            // smoothstep(1, 0.95, rrel)
-            const float tt = clampn(cl_div(rel - 1.0f, 0.95f - 1.0f), 0.0f, 1.0f);
-            const float ri = tt * tt * (3.0f - 2.0f * tt);
-            float ai = w[0] * at.inten[0] + w[1] * at.inten[1] + w[2] * at.inten[2] +
-                       w[3] * at.inten[3];
-            float ci = w[0] * at.refl[0] + w[1] * at.refl[1] + w[2] * at.refl[2] +
-                       w[3] * at.refl[3];
-            ai = fmaxf(ai, 0.0f);
-            ci = fmaxf(ci, 0.0f);
-            const float frag = 0.0f + gi * ri * ai * ci * (float)hits;
-            r0 += at.x * frag;
-            r1 += at.y * frag;
-            r2 += at.z * frag;
+        const uint64_t m = __ballot(hits > 0);
+        if (m == 0) continue;
+        const uint32_t n = (uint32_t)__popcll(m);
+        if (qlen + n > 64) flush();
+        if (hits > 0) s_q[qlen + lanes_below(m)] = (uint32_t)lane | (uint32_t)hits << 6 | nslot << 10;
+        if ((uint32_t)lane == nslot) {
+            slot_mask = m;
+            slot_lo = qlen;
+            slot_e = e;
         }
+        qlen += n;
+        nslot++;
     }
+    if (qlen > 0) flush();
     if (!inside) return;
     float4 out = make_float4(0, 0, 0, 0);
     if (r0 > 0 || r1 > 0 || r2 > 0) {
@@ -807,7 +902,8 @@ static int draw_impl(const float4* vtx, const float4* bounds, const float* ghost
                      const RasterPtrs& r, float4* image, hipStream_t s) {
     const long ntiles = (long)G.ntx * G.nty;
     (void)vtx;
-    hipLaunchKernelGGL(k_raster, dim3((unsigned)ntiles), dim3(64), 0, s, bounds, ghost, r.setup,
+    (void)ntiles;
+    hipLaunchKernelGGL(k_raster, dim3((unsigned)raster_grid(G)), dim3(64), 0, s, bounds, ghost, r.setup,
                        r.attr, r.vals, r.tstart, r.tend, r.h, G, image);
     RF_LAUNCH_CHECK("rasterizer");
     return RF_OK;
