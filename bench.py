@@ -69,9 +69,25 @@ def trace_flops(pipe) -> float:
 
 def raster_bytes(pipe) -> float:
     """Minimum bytes the rasterizer moves per frame: each vertex's 28 useful
-    bytes read once, the RGBA fp32 image written once."""
+    bytes and each quad's bounds (16 bytes) read once, the RGBA fp32 image
+    written once."""
     w, h = pipe.resolution
-    return pipe.P * pipe.R * pipe.W * 28.0 + w * h * 16.0
+    return pipe.P * pipe.R * pipe.W * 28.0 + pipe.P * pipe.Q * 16.0 + w * h * 16.0
+
+
+def measured_traffic(kernel: str = 'k_raster') -> float | None:
+    """HBM bytes per dispatch of `kernel` on the full workload, as the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary records them
+    (counter values scaled by the copy-kernel calibration)."""
+    path = os.path.join(ROOT, 'profiles', 'hbm_traffic_full_frame.txt')
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith(f'{kernel}_traffic_bytes:'):
+                    return float(line.split(':', 1)[1])
+    except OSError:
+        return None
+    return None
 
 
 def cpu_baseline(pipe, inp_name: str = 'full') -> dict:
@@ -270,9 +286,12 @@ def main() -> int:
     else:
         nbytes = raster_bytes(pipe)
         achieved = nbytes / (stages[dominant] * 1e-3) / 1e9
+        traffic = measured_traffic() if (dominant == 'rasterizer' and args.workload == 'full') else None
         roofline = {'kernel': 'k_raster (rf_raster_draw)' if dominant == 'rasterizer' else dominant,
                     'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                    'frac': achieved / HBM_PEAK_GBS, 'traffic': None}
+                    'frac': achieved / HBM_PEAK_GBS, 'traffic': traffic,
+                    'algorithmic_bytes': nbytes,
+                    'traffic_source': 'profiles/hbm_traffic_full_frame.txt' if traffic else None}
     trace_tflops = trace_flops(pipe) / (stages['raytrace'] * 1e-3) / 1e12
 
     line = {
@@ -299,6 +318,9 @@ def main() -> int:
         'trace_rays_per_s': pipe.rays / (stages['raytrace'] * 1e-3),
         'trace_tflops': trace_tflops,
         'roofline': roofline,
+        'trace_roofline': {'kernel': 'k_trace (rf_trace_planes)', 'bound': 'valu',
+                           'achieved': trace_tflops, 'peak': FP32_PEAK_TFLOPS, 'unit': 'TFLOP/s',
+                           'frac': trace_tflops / FP32_PEAK_TFLOPS},
     }
     if args.workload == 'full':
         line['adjoint'] = adjoint_timing(wl, args.steps)

@@ -13,9 +13,12 @@ the render path in ``realflare_amd`` never does.
   fractional Fourier transform (``realflare/api/tasks/ghost.py:13-36``,
   ``realflare/utils/frft.py:8-62``).
 
-Pinning: every function here is checked by ``tests/test_oracle*.py`` against
-the reference test vectors, closed forms, invariants, brute force or golden
-fixtures; the header of each test names which.
+Pinning: ``tests/test_oracle.py`` and ``tests/test_host.py`` check these
+functions against the reference's test vectors, closed forms, invariants,
+brute force or golden fixtures; the header of each test names which.  Parity
+unpinned (no reference fixture or closed form checks them): ``starburst``, the
+grating / scratches / dust aperture masks, and the prim / vertex shaders
+beyond what the rasterizer checks exercise.
 """
 from __future__ import annotations
 

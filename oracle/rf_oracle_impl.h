@@ -7,7 +7,9 @@
  * path links it.
  *
  * Each function cites the reference source it restates (paths relative to
- * the realflare repository).  Behaviour the reference leaves undefined is
+ * the realflare repository).  Parity unpinned (no reference fixture or closed
+ * form checks them): starburst, the grating / scratches / dust masks, and
+ * prim_shader / vertex_shader beyond what the rasterizer checks exercise.  Behaviour the reference leaves undefined is
  * fixed here, identically to the HIP kernels:
  *   - float -> int conversion truncates toward zero, saturates, NaN -> 0;
  *   - integer edge functions wrap modulo 2^32;
