@@ -221,6 +221,95 @@ def adjoint_timing(wl, steps: int) -> dict:
     return out
 
 
+def batched_timing(steps: int, warmup: int, n_frames: int = 64, tune_frames: int = 8,
+                   plain_frames: int = 4) -> dict:
+    """Batched configuration: the 12-surface lens, `n_frames` light positions
+    on a seeded spiral, per-ghost grids 32^2..256^2 from a 3x3 footprint
+    pre-trace, 16 wavelengths over 400-700 nm, 2048x2048 FLARE image per frame.
+    The batch (one persistent trace launch, prim/vertex passes, binning and
+    rasterization in passes of frames) is captured into one HIP graph and
+    replayed; launch choices come from the autotuner run on `tune_frames`
+    frames of the batch.  The plain per-ghost form (one trace and one
+    rasterization per ghost and frame) is timed on `plain_frames` frames."""
+    import numpy as np
+    import torch
+
+    from realflare_amd import autotune
+    from realflare_amd.batch import (BatchRenderer, PlainPerGhost, Tuning, footprint_grids,
+                                     spiral_positions)
+    from realflare_amd.synthetic import workload
+
+    wl = workload('full')
+    wavelengths = [int(round(400 + 300 * i / 15)) for i in range(16)]
+    positions = spiral_positions(n_frames, seed=7)
+    grids = footprint_grids(wl.project, wl.lens_model, wl.paths, positions)
+    idx = np.unique(np.linspace(0, n_frames - 1, tune_frames).round().astype(int))
+    sample = BatchRenderer(wl.project, wl.lens_model, wl.paths, wavelengths,
+                           [positions[i] for i in idx], grids[idx], Tuning())
+    tuning = autotune.tune(sample, reps=2, key='batched-12s-66g-16w-2048px')
+    record = sample.autotune_record
+    del sample
+    torch.cuda.empty_cache()
+    print(f'# autotuned: {tuning}', file=sys.stderr, flush=True)
+
+    br = BatchRenderer(wl.project, wl.lens_model, wl.paths, wavelengths, positions, grids, tuning)
+    pairs = br.fit_capacity()
+    graph = br.capture()
+    for _ in range(warmup):
+        graph.replay()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(steps):
+        graph.replay()
+    b.record()
+    torch.cuda.synchronize()
+    ms_batch = a.elapsed_time(b) / steps
+    # stage split of one eager batch
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    ev[0].record()
+    br.trace()
+    ev[1].record()
+    br.shade()
+    ev[2].record()
+    for d in br.passes:
+        br.raster_pass(d)
+    ev[3].record()
+    torch.cuda.synchronize()
+    split = {'trace_ms': ev[0].elapsed_time(ev[1]), 'prim_vertex_ms': ev[1].elapsed_time(ev[2]),
+             'bin_raster_ms': ev[2].elapsed_time(ev[3])}
+    batch_images = br.images
+
+    plain = PlainPerGhost(br)
+    frames = [int(f) for f in np.linspace(0, n_frames - 1, plain_frames).round()]
+    plain.fit_capacity(frames)
+    plain.render_frame(frames[0])
+    torch.cuda.synchronize()
+    a.record()
+    for f in frames:
+        plain.render_frame(f)
+    b.record()
+    torch.cuda.synchronize()
+    ms_plain = a.elapsed_time(b) / len(frames)
+    errs = []
+    for f in frames:
+        d = (plain.images[f] - batch_images[f]).abs().sum()
+        errs.append(float(d / batch_images[f].abs().sum().clamp(min=1e-30)))
+    ms_frame = ms_batch / n_frames
+    return {
+        'frames': n_frames, 'wavelengths': len(wavelengths),
+        'grid_min': int(grids.min()), 'grid_median': float(np.median(grids)),
+        'grid_max': int(grids.max()), 'rays_per_batch': br.rays,
+        'rays_per_frame': br.rays / n_frames, 'max_pass_pairs': pairs,
+        'ms_per_batch': ms_batch, 'ms_per_frame': ms_frame, 'frames_per_s': 1000.0 / ms_frame,
+        'rays_per_s': br.rays / (ms_batch * 1e-3), 'stage_split_eager': split,
+        'hipgraph': True, 'tuning': record,
+        'plain_per_ghost_ms_per_frame': ms_plain, 'plain_frames_timed': frames,
+        'speedup_vs_plain_per_ghost': ms_plain / ms_frame,
+        'batched_vs_plain_rel_l1_max': max(errs),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -228,6 +317,8 @@ def main() -> int:
     ap.add_argument('--warmup', type=int, default=3)
     ap.add_argument('--workload', default='full', choices=['full', 'small'])
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--no-batched', action='store_true', help='skip the batched configuration')
+    ap.add_argument('--batch-frames', type=int, default=64)
     ap.add_argument('--copy-calibration', action='store_true',
                     help='also run a device-to-device copy of 256 MiB (for counter scaling)')
     args = ap.parse_args()
@@ -325,6 +416,9 @@ def main() -> int:
     if args.workload == 'full':
         line['adjoint'] = adjoint_timing(wl, args.steps)
         print(f"# adjoint: {line['adjoint']}", file=sys.stderr, flush=True)
+    if args.workload == 'full' and not args.no_batched:
+        line['batched'] = batched_timing(max(1, min(args.steps, 3)), 1, args.batch_frames)
+        print(f"# batched: {line['batched']}", file=sys.stderr, flush=True)
     if args.copy_calibration:
         src = torch.empty((64 << 20,), dtype=torch.float32, device='cuda')
         dst = torch.empty_like(src)

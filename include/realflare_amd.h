@@ -140,6 +140,64 @@ int rf_trace_backward(const rf_lens_element *lens, const float *coating_lambda,
                       const float *grad_pos_y, const float *grad_reflectance, void *workspace,
                       size_t workspace_bytes, float *grad_out, rf_stream_t stream);
 
+/*
+ * Batched form.  No counterpart in the reference, which renders one light
+ * position with one grid size for every ghost: a batch holds many frames
+ * (light positions), each ghost with its own grid, traced in one persistent
+ * launch and rasterized per group of frames.
+ *
+ * A segment is one (frame, ghost) pair.  Its rays are stored contiguously from
+ * ray_offset, wavelength-major ([W][g*g], the layout of one ghost of
+ * rf_trace_planes), its vertices likewise from ray_offset ([g*g][W], the layout
+ * of one ghost of rf_vertex_shader), its quads from quad_offset ((g-1)^2 of
+ * them).  Segments are ordered by frame, then ghost; offsets increase.
+ */
+typedef struct rf_batch_segment {
+    int64_t ray_offset;
+    int64_t quad_offset;
+    int32_t frame;        /* output frame */
+    int32_t path;         /* index into the paths array */
+    int32_t lens;         /* index into the lens table */
+    int32_t grid_count;   /* g of this ghost, >= 2 */
+    float grid_length;
+    float direction[3];   /* the frame's light direction (update_direction) */
+} rf_batch_segment;       /* 48 bytes */
+
+/* A run of consecutive segments (all of them, or the frames of one raster
+ * pass).  segments is a DEVICE pointer to the run's first segment. */
+typedef struct rf_batch {
+    const rf_batch_segment *segments;
+    int32_t segment_count;
+    int32_t frame_base;   /* frame of the run's first segment */
+    int32_t frame_count;  /* frames frame_base .. frame_base + frame_count - 1 */
+    int32_t reserved_;
+    int64_t ray_base;     /* ray_offset of segments[0] */
+    int64_t ray_count;    /* rays of the run (W * g^2 summed) */
+    int64_t quad_base;    /* quad_offset of segments[0] */
+    int64_t quad_count;   /* quads of the run */
+    int64_t ray_total;    /* rays of the whole trace output (the plane stride) */
+} rf_batch;
+
+/*
+ * Trace every segment of `batch` in one persistent launch: `workgroups`
+ * workgroups of 256 lanes take chunks of 256 * rays_per_lane rays of one
+ * (segment, wavelength) from a queue until it is empty.
+ *   lenses       rf_lens_element records of all lens instances, concatenated
+ *   lens_table   int32[instances][3]: first element, lens_count, aperture_index
+ *   chunk_offsets uint64[segment_count + 1] (device): exclusive prefix sum of
+ *                W * ceil(g^2 / (256 * rays_per_lane)) over the segments
+ *   workspace    >= rf_trace_batch_workspace_size() bytes (the queue head; it is
+ *                reset by the call, on the stream)
+ *   planes       float[6][ray_total], as rf_trace_planes per segment
+ * Same per-ray arithmetic as rf_trace_planes.
+ */
+size_t rf_trace_batch_workspace_size(void);
+int rf_trace_batch(const rf_lens_element *lenses, const int32_t *lens_table,
+                   const int32_t *paths, const int32_t *wavelengths, int32_t wavelength_count,
+                   float coating_min_ior, const rf_batch *batch, const uint64_t *chunk_offsets,
+                   int32_t rays_per_lane, int32_t workgroups, void *workspace,
+                   size_t workspace_bytes, float *planes, rf_stream_t stream);
+
 /* Settings of the rasterizer kernels (rasterizing.cl:233-663), as
  * RasterizingTask.rasterize derives them (rasterizing.py:296-421). */
 typedef struct rf_raster_params {
@@ -155,6 +213,10 @@ typedef struct rf_raster_params {
     float ghost_scale;            /* 1 - lens.fstop / 32 (py:393) */
     int32_t ghost_width, ghost_height;
     int32_t spectrum_count;       /* texels of the light spectrum, RGBA each */
+    int32_t accumulate;           /* 0: write the image (the reference); 1: add into it */
+    int32_t super_block;          /* 8x8-pixel tiles per side of the groups the rasterizer
+                                     keeps on one XCD: 1, 2, 4 or 8; 0 = 4.  No effect on
+                                     the result. */
     int32_t reserved_;
 } rf_raster_params;
 
@@ -227,6 +289,36 @@ int rf_flare(const float *planes, const float *ghost, const float *light_spectru
  */
 int rf_raster_status(const void *workspace, uint64_t *required_pairs, int *overflow,
                      rf_stream_t stream);
+
+/*
+ * Batched rasterization of a run of segments (see rf_batch).  prm supplies
+ * the frame settings; its path_count, grid_count and area_orig are ignored,
+ * and min_area holds lens.min_area unscaled: each segment's area_orig is
+ * (grid_length / (g - 1))^2 (rasterizing.py:152-156) and its minimum area
+ * min_area * area_orig (py:329).  Arrays are indexed as the segments say:
+ *   planes    float[6][ray_total] (rf_trace_batch)
+ *   bounds    float4[quads], intensities float[quads][W], by global quad index
+ *   vertexes  Vertex[rays] by global ray index ([g*g][W] per segment)
+ *   images    float4[frame_count][height][width], the run's frames in order
+ * The prim / vertex passes cover the run's quads / rays; the binner and the
+ * rasterizer bin (frame, tile) pairs of all the run's frames together, with
+ * rf_raster_status reporting pairs and overflow as for one frame.
+ */
+int rf_prim_shader_batch(const float *planes, const rf_raster_params *prm, const rf_batch *batch,
+                         float *bounds, float *intensities, rf_stream_t stream);
+int rf_vertex_shader_batch(const float *planes, const float *intensities,
+                           const rf_raster_params *prm, const rf_batch *batch, void *vertexes,
+                           rf_stream_t stream);
+size_t rf_raster_batch_workspace_size(const rf_raster_params *prm, const rf_batch *batch,
+                                      uint64_t pair_capacity);
+int rf_raster_bin_batch(const void *vertexes, const float *bounds, const float *light_spectrum,
+                        const rf_raster_params *prm, const rf_batch *batch,
+                        uint64_t pair_capacity, void *workspace, size_t workspace_bytes,
+                        rf_stream_t stream);
+int rf_raster_draw_batch(const void *vertexes, const float *bounds, const float *ghost,
+                         const rf_raster_params *prm, const rf_batch *batch,
+                         uint64_t pair_capacity, void *workspace, size_t workspace_bytes,
+                         float *images, rf_stream_t stream);
 
 /* Aperture settings as ApertureTask.aperture passes them to its kernels
  * (aperture.py:80-153): angles in radians, softness / 10, widths and the

@@ -39,6 +39,8 @@ class RasterParams(ctypes.Structure):
         ('ghost_width', ctypes.c_int32),
         ('ghost_height', ctypes.c_int32),
         ('spectrum_count', ctypes.c_int32),
+        ('accumulate', ctypes.c_int32),
+        ('super_block', ctypes.c_int32),
         ('reserved_', ctypes.c_int32),
     ]
 

@@ -38,7 +38,79 @@ struct Geom {
     float st, area_orig, min_area, intensity, ghost_scale;
     int gw, gh, nspec;
     int bin_exact;
+    int accumulate;  // add into the image instead of writing it
+    int sb;          // super-block edge in tiles (XCD grouping of k_raster)
+    // Batched runs (batch != 0): the run's segments.  Plain frames use the
+    // uniform P x g layout with one frame.
+    int batch, nseg, frame_base, frames;
+    const rf_batch_segment* segs;
+    long ray_base, quad_base;  // first ray / quad of the run
+    long NQ;                   // quads of this pass (plain: P*Q)
+    long NR;                   // vertices of this pass (plain: N)
+    float min_area_factor;     // batch: lens.min_area (scaled per segment)
 };
+
+// Where a ghost's data lives: first ray (planes [W][R], vertices [R][W]),
+// first global quad, grid, and the frame relative to the run.
+struct Loc {
+    long ray0, quad0;
+    int g, R, Q, frame;
+    float area_orig, min_area;
+};
+
+__device__ __forceinline__ Loc loc_segment(const Geom& G, int s) {
+    const rf_batch_segment sg = G.segs[s];
+    Loc l;
+    l.ray0 = sg.ray_offset;
+    l.quad0 = sg.quad_offset;
+    l.g = sg.grid_count;
+    l.R = l.g * l.g;
+    l.Q = (l.g - 1) * (l.g - 1);
+    l.frame = sg.frame - G.frame_base;
+    // rasterizing.py:152-156 and :329, per segment
+    const double d = (double)sg.grid_length / (double)(l.g - 1);
+    l.area_orig = (float)(d * d);
+    l.min_area = (float)((double)G.min_area_factor * (double)l.area_orig);
+    return l;
+}
+
+__device__ __forceinline__ Loc loc_ghost(const Geom& G, int path) {
+    Loc l;
+    l.ray0 = (long)path * G.W * G.R;
+    l.quad0 = (long)path * G.Q;
+    l.g = G.g;
+    l.R = G.R;
+    l.Q = G.Q;
+    l.frame = 0;
+    l.area_orig = G.area_orig;
+    l.min_area = G.min_area;
+    return l;
+}
+
+// segment holding global quad gq (largest s with quad_offset <= gq)
+__device__ __forceinline__ int seg_by_quad(const Geom& G, long gq) {
+    int lo = 0, hi = G.nseg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (G.segs[mid].quad_offset <= gq) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ int seg_by_ray(const Geom& G, long gr) {
+    int lo = 0, hi = G.nseg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (G.segs[mid].ray_offset <= gr) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ Loc loc_quad(const Geom& G, long gq) {
+    return G.batch ? loc_segment(G, seg_by_quad(G, gq)) : loc_ghost(G, (int)(gq / G.Q));
+}
 
 struct Header {
     uint64_t required;  // (tile, quad-step) pairs the frame needs
@@ -104,11 +176,13 @@ __global__ __launch_bounds__(256) void k_prim(const float* __restrict__ planes, 
                                               float4* __restrict__ bounds,
                                               float* __restrict__ intens) {
 #pragma clang fp contract(off)
-    const long idx = (long)blockIdx.x * 256 + threadIdx.x;
-    if (idx >= (long)G.P * G.Q) return;
-    const int path = (int)(idx / G.Q), quad = (int)(idx % G.Q);
+    const long t = (long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= G.NQ) return;
+    const long idx = G.quad_base + t;  // global quad
+    const Loc l = loc_quad(G, idx);
+    const int quad = (int)(idx - l.quad0);
     int qv[4];
-    quad_vertexes(G.g, quad, qv);
+    quad_vertexes(l.g, quad, qv);
     const float* px_ = planes;
     const float* py_ = planes + G.N;
     const float* rr_ = planes + 4 * G.N;
@@ -121,7 +195,7 @@ __global__ __launch_bounds__(256) void k_prim(const float* __restrict__ planes, 
             *out_i = 0.0f;
             continue;
         }
-        const long ro = ((long)path * G.W + w) * G.R;
+        const long ro = l.ray0 + (long)w * l.R;
         float rx[4], ry[4], rrel[4], refl[4];
 #pragma unroll
         for (int i = 0; i < 4; i++) {
@@ -159,8 +233,8 @@ __global__ __launch_bounds__(256) void k_prim(const float* __restrict__ planes, 
             continue;
         }
         if (rrel[0] > 1 && rrel[1] > 1 && rrel[2] > 1 && rrel[3] > 1) invalid_rrel++;
-        float area_actual = fmaxf(area, G.min_area);
-        *out_i = area_actual > 0 ? G.area_orig / area_actual : 0.0f;
+        float area_actual = fmaxf(area, l.min_area);
+        *out_i = area_actual > 0 ? l.area_orig / area_actual : 0.0f;
         b0 = fminf(b0, fminf(fminf(cx[0], cx[1]), fminf(cx[2], cx[3])));
         b1 = fminf(b1, fminf(fminf(cy[0], cy[1]), fminf(cy[2], cy[3])));
         b2 = fmaxf(b2, fmaxf(fmaxf(cx[0], cx[1]), fmaxf(cx[2], cx[3])));
@@ -176,27 +250,30 @@ __global__ __launch_bounds__(256) void k_vertex(const float* __restrict__ planes
                                                 const float* __restrict__ intens, Geom G,
                                                 float4* __restrict__ vertexes) {
 #pragma clang fp contract(off)
-    const long idx = (long)blockIdx.x * 256 + threadIdx.x;
-    if (idx >= G.NV) return;
-    const int w = (int)(idx % G.W);
-    const int ray = (int)((idx / G.W) % G.R);
-    const int path = (int)(idx / ((long)G.W * G.R));
-    const long ri = ((long)path * G.W + w) * G.R + ray;
+    const long t = (long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= G.NR) return;
+    const long idx = G.ray_base + t;  // global vertex
+    const Loc l = G.batch ? loc_segment(G, seg_by_ray(G, idx))
+                          : loc_ghost(G, (int)(idx / ((long)G.W * G.R)));
+    const long local = idx - l.ray0;
+    const int w = (int)(local % G.W);
+    const int ray = (int)(local / G.W);
+    const long ri = l.ray0 + (long)w * l.R + ray;
     const float refl = planes[5 * G.N + ri];
     float4 a = make_float4(0, 0, 0, 0), b = make_float4(0, refl, 0, 0);
     if (!isnan(refl)) {
-        const int x = ray % G.g, y = ray / G.g;
+        const int x = ray % l.g, y = ray / l.g;
         int nb[4];
-        nb[0] = (y - 1) * (G.g - 1) + (x - 1);
+        nb[0] = (y - 1) * (l.g - 1) + (x - 1);
         nb[1] = nb[0] + 1;
-        nb[2] = y * (G.g - 1) + (x - 1);
+        nb[2] = y * (l.g - 1) + (x - 1);
         nb[3] = nb[2] + 1;
         float it = 0.0f;
         int count = 0;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            if (nb[i] < 0 || nb[i] >= G.Q) continue;
-            float v = intens[((long)path * G.Q + nb[i]) * G.W + w];
+            if (nb[i] < 0 || nb[i] >= l.Q) continue;
+            float v = intens[(l.quad0 + nb[i]) * G.W + w];
             if (v > 0) {
                 it += v;
                 ++count;
@@ -328,7 +405,7 @@ __global__ __launch_bounds__(256) void k_setup(const float4* __restrict__ vtx,
 #pragma clang fp contract(off)
     const long e = (long)blockIdx.x * 256 + threadIdx.x;
     if (e >= G.NE) return;
-    const long prim = e / G.S;
+    const long prim = G.quad_base + e / G.S;  // global quad
     const int step = (int)(e % G.S);
     const int wl = step / G.Wsub;
     uint32_t count = 0;
@@ -339,16 +416,17 @@ __global__ __launch_bounds__(256) void k_setup(const float4* __restrict__ vtx,
     rec.prim = (uint32_t)prim;
     for (int j = 0; j < 4; j++) rec.vx[j] = rec.vy[j] = 0;
     if (!isnan(b.x)) {
-        const int path = (int)(prim / G.Q), quad = (int)(prim % G.Q);
+        const Loc l = loc_quad(G, prim);
+        const int quad = (int)(prim - l.quad0);
         int qv[4];
-        quad_vertexes(G.g, quad, qv);
+        quad_vertexes(l.g, quad, qv);
         const float4 tab = table[step];
         const float blend = tab.x;
         int vx[4], vy[4];
         Attr at;
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            const long base = ((long)path * G.R + qv[j]) * G.W + wl;
+            const long base = l.ray0 + (long)qv[j] * G.W + wl;
             const float4 a0 = vtx[2 * base], a1 = vtx[2 * base + 1];
             float4 b0 = make_float4(0, 0, 0, 0), b1 = make_float4(0, 0, 0, 0);
             if (base + 1 < G.NV) {
@@ -449,13 +527,15 @@ __global__ __launch_bounds__(256) void k_expand(const Setup* __restrict__ setup,
     HullRows hull;
     hull_prepare(rec.vx, rec.vy, &hull);
     uint64_t off = offsets[e];
+    const uint32_t frame_key =
+        G.batch ? (uint32_t)loc_quad(G, rec.prim).frame * (uint32_t)(G.ntx * G.nty) : 0u;
     for (uint32_t ty = ty0; ty <= ty1; ty++) {
         long a, b;
         hull_row_span(hull, G.AA, (int)ty, &a, &b);
         a = max(a, (long)tx0);
         b = min(b, (long)tx1);
         for (long tx = a; tx <= b; tx++) {
-            keys[off] = ty * (uint32_t)G.ntx + (uint32_t)tx;
+            keys[off] = frame_key + ty * (uint32_t)G.ntx + (uint32_t)tx;
             vals[off] = (uint32_t)e;
             off++;
         }
@@ -581,19 +661,20 @@ __device__ __forceinline__ void barycentric_fast(int px, int py, const int* x, c
 // Workgroups are dealt to the XCDs round-robin (workgroup i runs on XCD
 // i % RF_XCDS) and each XCD has its own L2.  A quad's records are read by every
 // tile it overlaps, so neighbouring tiles should share an L2: the screen is cut
-// into SB x SB-tile super-blocks, super-block b goes to XCD b % RF_XCDS (which
-// keeps the XCDs balanced), and an XCD walks its super-blocks in order, the
-// SB * SB tiles of one super-block back to back.
+// into SB x SB-tile super-blocks (SB = G.sb), super-block b goes to XCD
+// b % RF_XCDS (which keeps the XCDs balanced), and an XCD walks its
+// super-blocks in order, the SB * SB tiles of one super-block back to back.
 static constexpr int RF_XCDS = 8;
-static constexpr int SB = 4;
 
 __host__ __device__ __forceinline__ long raster_grid(const Geom& G) {
+    const int SB = G.sb;
     const long nsb = (long)((G.ntx + SB - 1) / SB) * ((G.nty + SB - 1) / SB);
     const long per_xcd = (nsb + RF_XCDS - 1) / RF_XCDS;
     return per_xcd * RF_XCDS * SB * SB;
 }
 
 __device__ __forceinline__ bool raster_tile(unsigned bid, const Geom& G, int* tx, int* ty) {
+    const int SB = G.sb;
     const unsigned xcd = bid % RF_XCDS, j = bid / RF_XCDS;
     const unsigned sb = (j / (SB * SB)) * RF_XCDS + xcd, t = j % (SB * SB);
     const int nsbx = (G.ntx + SB - 1) / SB;
@@ -654,9 +735,11 @@ __global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ bounds
 #pragma clang fp contract(off)
     __shared__ uint32_t s_q[64];  // pixel lane | hits << 6 | slot << 10
     if (h->overflow) return;
+    const long per_frame = raster_grid(G);
+    const int frame = (int)(blockIdx.x / per_frame);
     int tx, ty;
-    if (!raster_tile(blockIdx.x, G, &tx, &ty)) return;
-    const int tile = ty * G.ntx + tx;
+    if (!raster_tile((unsigned)(blockIdx.x - frame * per_frame), G, &tx, &ty)) return;
+    const int tile = frame * G.ntx * G.nty + ty * G.ntx + tx;
     const int lane = threadIdx.x;
     const int x = tx * TILE + (lane & (TILE - 1)), y = ty * TILE + (lane / TILE);
     const bool inside = x < G.width && y < G.height;
@@ -747,7 +830,12 @@ __global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ bounds
         xyz_to_ap1(r0 * f, r1 * f, r2 * f, o);
         out = make_float4(o[0], o[1], o[2], o[3]);
     }
-    image[(long)(G.height - 1 - y) * G.width + x] = out;
+    float4* dst = image + (long)frame * G.width * G.height + (long)(G.height - 1 - y) * G.width + x;
+    if (G.accumulate) {
+        const float4 prev = *dst;
+        out = make_float4(prev.x + out.x, prev.y + out.y, prev.z + out.z, prev.w + out.w);
+    }
+    *dst = out;
 }
 
 // ---------------------------------------------------------------------------
@@ -795,6 +883,21 @@ static int make_geom(const rf_raster_params* p, Geom* G) {
     G->gh = p->ghost_height;
     G->nspec = p->spectrum_count;
     G->bin_exact = (G->bin % TILE) == 0;
+    G->accumulate = p->accumulate != 0;
+    G->sb = p->super_block == 0 ? 4 : p->super_block;
+    if (!(G->sb == 1 || G->sb == 2 || G->sb == 4 || G->sb == 8))
+        return set_error(RF_ERR_INVALID, "raster: super_block must be 0, 1, 2, 4 or 8 (got %d)",
+                         p->super_block);
+    G->batch = 0;
+    G->nseg = 0;
+    G->frame_base = 0;
+    G->frames = 1;
+    G->segs = nullptr;
+    G->ray_base = 0;
+    G->quad_base = 0;
+    G->NQ = (long)G->P * G->Q;
+    G->NR = G->N;
+    G->min_area_factor = 0.0f;
     if (G->W > 1 && G->nspec < 1)
         return set_error(RF_ERR_INVALID, "raster: spectrum_count must be >= 1");
     if (G->NE > 0xffffffffL)
@@ -827,7 +930,7 @@ static Layout layout(const Geom& G, uint64_t cap) {
     L.keys = take(cap * sizeof(uint32_t));
     L.vals = take(cap * sizeof(uint32_t));
     L.radix_ws = take(radix_workspace_bytes(cap));
-    const size_t ntiles = (size_t)G.ntx * G.nty;
+    const size_t ntiles = (size_t)G.ntx * G.nty * G.frames;
     L.tstart = take(ntiles * sizeof(uint32_t));
     L.tend = take(ntiles * sizeof(uint32_t));
     L.total = o;
@@ -869,7 +972,7 @@ static int raster_ptrs(const Geom& G, uint64_t cap, void* ws, size_t ws_bytes, R
 // binner: step tables, quad-step setup, pair expansion, stable sort, tile ranges
 static int bin_impl(const float4* vtx, const float4* bounds, const float* spectrum, const Geom& G,
                     uint64_t cap, const RasterPtrs& r, hipStream_t s) {
-    const long ntiles = (long)G.ntx * G.nty;
+    const long ntiles = (long)G.ntx * G.nty * G.frames;
     hipLaunchKernelGGL(k_steps, dim3(blocks_for(G.maxw, 64)), dim3(64), 0, s, spectrum, G, r.table);
     if (G.NE > 0) {
         hipLaunchKernelGGL(k_setup, dim3(blocks_for(G.NE, 256)), dim3(256), 0, s, vtx, bounds,
@@ -903,7 +1006,8 @@ static int draw_impl(const float4* vtx, const float4* bounds, const float* ghost
     const long ntiles = (long)G.ntx * G.nty;
     (void)vtx;
     (void)ntiles;
-    hipLaunchKernelGGL(k_raster, dim3((unsigned)raster_grid(G)), dim3(64), 0, s, bounds, ghost, r.setup,
+    hipLaunchKernelGGL(k_raster, dim3((unsigned)(raster_grid(G) * G.frames)), dim3(64), 0, s, bounds,
+                       ghost, r.setup,
                        r.attr, r.vals, r.tstart, r.tend, r.h, G, image);
     RF_LAUNCH_CHECK("rasterizer");
     return RF_OK;
@@ -1053,6 +1157,112 @@ extern "C" int rf_flare(const float* planes, const float* ghost, const float* li
     }
     return raster_impl(vtx, bounds, ghost, light_spectrum, G, pair_capacity, w + F.raster,
                        F.bounds - F.raster, (float4*)image, s);
+}
+
+// ---------------------------------------------------------------------------
+// Batched runs
+
+static int make_geom_batch(const rf_raster_params* p, const rf_batch* b, const char* who,
+                           Geom* G) {
+    if (!p || !b) return set_error(RF_ERR_INVALID, "%s: NULL params or batch", who);
+    rf_raster_params q = *p;
+    q.path_count = 0;  // the segments give ghosts and grids
+    q.grid_count = 2;
+    int rc = make_geom(&q, G);
+    if (rc) return rc;
+    if (b->segment_count < 1 || !b->segments)
+        return set_error(RF_ERR_INVALID, "%s: empty segment list", who);
+    if (b->frame_count < 1 || b->frame_count > 65535 ||
+        (long)b->frame_count * G->ntx * G->nty > 0xffffffffL)
+        return set_error(RF_ERR_INVALID, "%s: frame_count %d", who, b->frame_count);
+    if (b->ray_base < 0 || b->ray_count < 0 || b->quad_base < 0 || b->quad_count < 0 ||
+        b->ray_base + b->ray_count > b->ray_total)
+        return set_error(RF_ERR_INVALID, "%s: inconsistent ray / quad ranges", who);
+    G->batch = 1;
+    G->segs = b->segments;
+    G->nseg = b->segment_count;
+    G->frame_base = b->frame_base;
+    G->frames = b->frame_count;
+    G->ray_base = b->ray_base;
+    G->quad_base = b->quad_base;
+    G->NQ = b->quad_count;
+    G->NR = b->ray_count;
+    G->N = b->ray_total;  // plane stride
+    G->NV = b->ray_total;  // vertices of the whole trace
+    G->NE = b->quad_count * G->S;
+    G->min_area_factor = p->min_area;
+    if (G->NE > 0xffffffffL)
+        return set_error(RF_ERR_INVALID, "%s: %ld quad steps exceed 2^32", who, G->NE);
+    return RF_OK;
+}
+
+extern "C" int rf_prim_shader_batch(const float* planes, const rf_raster_params* prm,
+                                    const rf_batch* batch, float* bounds, float* intensities,
+                                    rf_stream_t stream) {
+    Geom G;
+    int rc = make_geom_batch(prm, batch, "rf_prim_shader_batch", &G);
+    if (rc) return rc;
+    if (!planes || !bounds || !intensities)
+        return set_error(RF_ERR_INVALID, "rf_prim_shader_batch: NULL");
+    if (G.NQ == 0) return RF_OK;
+    hipLaunchKernelGGL(k_prim, dim3(blocks_for(G.NQ, 256)), dim3(256), 0, (hipStream_t)stream,
+                       planes, G, (float4*)bounds, intensities);
+    RF_LAUNCH_CHECK("rf_prim_shader_batch");
+    return RF_OK;
+}
+
+extern "C" int rf_vertex_shader_batch(const float* planes, const float* intensities,
+                                      const rf_raster_params* prm, const rf_batch* batch,
+                                      void* vertexes, rf_stream_t stream) {
+    Geom G;
+    int rc = make_geom_batch(prm, batch, "rf_vertex_shader_batch", &G);
+    if (rc) return rc;
+    if (!planes || !intensities || !vertexes)
+        return set_error(RF_ERR_INVALID, "rf_vertex_shader_batch: NULL");
+    if (G.NR == 0) return RF_OK;
+    hipLaunchKernelGGL(k_vertex, dim3(blocks_for(G.NR, 256)), dim3(256), 0, (hipStream_t)stream,
+                       planes, intensities, G, (float4*)vertexes);
+    RF_LAUNCH_CHECK("rf_vertex_shader_batch");
+    return RF_OK;
+}
+
+extern "C" size_t rf_raster_batch_workspace_size(const rf_raster_params* prm,
+                                                 const rf_batch* batch, uint64_t pair_capacity) {
+    Geom G;
+    if (make_geom_batch(prm, batch, "rf_raster_batch_workspace_size", &G)) return 0;
+    return layout(G, pair_capacity).total;
+}
+
+extern "C" int rf_raster_bin_batch(const void* vertexes, const float* bounds,
+                                   const float* light_spectrum, const rf_raster_params* prm,
+                                   const rf_batch* batch, uint64_t pair_capacity, void* workspace,
+                                   size_t workspace_bytes, rf_stream_t stream) {
+    Geom G;
+    int rc = make_geom_batch(prm, batch, "rf_raster_bin_batch", &G);
+    if (rc) return rc;
+    if ((G.W > 1 && !light_spectrum) || (G.NE > 0 && (!vertexes || !bounds)))
+        return set_error(RF_ERR_INVALID, "rf_raster_bin_batch: NULL buffer");
+    RasterPtrs r;
+    rc = raster_ptrs(G, pair_capacity, workspace, workspace_bytes, &r, "rf_raster_bin_batch");
+    if (rc) return rc;
+    return bin_impl((const float4*)vertexes, (const float4*)bounds, light_spectrum, G,
+                    pair_capacity, r, (hipStream_t)stream);
+}
+
+extern "C" int rf_raster_draw_batch(const void* vertexes, const float* bounds, const float* ghost,
+                                    const rf_raster_params* prm, const rf_batch* batch,
+                                    uint64_t pair_capacity, void* workspace,
+                                    size_t workspace_bytes, float* images, rf_stream_t stream) {
+    Geom G;
+    int rc = make_geom_batch(prm, batch, "rf_raster_draw_batch", &G);
+    if (rc) return rc;
+    if (!images || !ghost || (G.NE > 0 && (!vertexes || !bounds)))
+        return set_error(RF_ERR_INVALID, "rf_raster_draw_batch: NULL buffer");
+    RasterPtrs r;
+    rc = raster_ptrs(G, pair_capacity, workspace, workspace_bytes, &r, "rf_raster_draw_batch");
+    if (rc) return rc;
+    return draw_impl((const float4*)vertexes, (const float4*)bounds, ghost, G, r, (float4*)images,
+                     (hipStream_t)stream);
 }
 
 extern "C" int rf_raster_status(const void* workspace, uint64_t* required_pairs, int* overflow,

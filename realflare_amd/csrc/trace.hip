@@ -108,51 +108,41 @@ __device__ __forceinline__ Hit intersect(f3 pos, f3 dir, float radius, float cen
     return i;
 }
 
+struct RayOut {
+    f3 pos, dir;
+    float apt_x, apt_y, rrel, refl;
+};
+
+// One ray of the `raytrace` kernel (raytracing.cl:178-359): grid ray `ray_id`
+// of a g x g grid through lens[0..L), ghost path (b1, b2), refractive indices
+// of the ray's wavelength in s_n.  Returns the final state; reflectance is NaN
+// for a ray that left the lens or totally reflected.  Callers keep the path,
+// wavelength and lens wave-uniform.
 template <int OUT>
-__global__ __launch_bounds__(256) void k_trace(const rf_lens_element* __restrict__ lens,
-                                               const int* __restrict__ paths,
-                                               const int* __restrict__ wavelengths,
-                                               rf_trace_params prm, void* __restrict__ out,
-                                               void* __restrict__ inters,
-                                               const float* __restrict__ coating_lambda) {
+__device__ __forceinline__ RayOut trace_ray(const rf_lens_element* __restrict__ lens, int L,
+                                            int aperture, int b1, int b2, float wavelength,
+                                            const float* s_n, int g, float grid_length,
+                                            const float* direction, float coating_min_ior,
+                                            const float* __restrict__ coating_lambda, int ray_id,
+                                            bool active, void* __restrict__ inters, long idx) {
 #pragma clang fp contract(off)
-    __shared__ float s_n[RF_LMAX];
-    const int L = prm.lens_count;
-    const int w = blockIdx.y, p = blockIdx.z;
-    const int g = prm.grid_count, R = g * g, W = prm.wavelength_count;
-    const float wavelength = (float)wavelengths[w];
-    const bool disperse = !isnan(lens[0].coefficients[0]);
-    if ((int)threadIdx.x < L) {
-        const rf_lens_element& e = lens[threadIdx.x];
-        float n = e.ior;
-        if (disperse && n > 1) n = dispersion(wavelength, e.coefficients);
-        s_n[threadIdx.x] = n;
-    }
-    __syncthreads();
-
-    const int ray_id = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool active = ray_id < R;
-    const long idx = ((long)p * W + w) * R + ray_id;
-
     // init_ray, raytracing.cl:178-215
     f3 pos, dir;
     {
         int y = (int)truncf((float)ray_id / (float)g);
         int x = ray_id - y * g;
-        float px = (float)((double)prm.grid_length *
+        float px = (float)((double)grid_length *
                            ((double)((float)x / (float)(g - 1)) - 0.5));
-        float py = (float)((double)prm.grid_length *
+        float py = (float)((double)grid_length *
                            (0.5 - (double)((float)y / (float)(g - 1))));
         Hit h = intersect<false>(mk3(px, py, 1.0f), mk3(0, 0, -1.0f), lens[0].radius,
                                  lens[0].center);
-        f3 d = norm3(mk3(-prm.direction[0], -prm.direction[1], -prm.direction[2]));
+        f3 d = norm3(mk3(-direction[0], -direction[1], -direction[2]));
         pos = sub3(h.pos, d);
         dir = d;
     }
     float apt_x = 0.0f, apt_y = 0.0f, rrel = 0.0f, refl = 1.0f;
 
-    const int b1 = paths[2 * p], b2 = paths[2 * p + 1];
-    const int aperture = prm.aperture_index;
     bool alive = active, died = false;
     int lens_id = 0, step = 0, delta = 1, inter_id = 0;
     const int inter_count = 3 * L - 1;
@@ -200,7 +190,7 @@ __global__ __launch_bounds__(256) void k_trace(const rf_lens_element* __restrict
                             float r;
                             if (coating_f > 0) {
                                 float lambda_c = coating_f;
-                                float nc = fmaxf(sqrtf(n1 * n2), prm.coating_min_ior);
+                                float nc = fmaxf(sqrtf(n1 * n2), coating_min_ior);
                                 float d1 = lambda_c / (4.0f * nc);
                                 r = fresnel_ar(theta, wavelength, d1, n1, nc, n2);
                             } else {
@@ -233,22 +223,60 @@ __global__ __launch_bounds__(256) void k_trace(const rf_lens_element* __restrict
         if (!__any(alive)) break;
     }
     if (died) refl = NAN;
+    RayOut o;
+    o.pos = pos;
+    o.dir = dir;
+    o.apt_x = apt_x;
+    o.apt_y = apt_y;
+    o.rrel = rrel;
+    o.refl = refl;
+    return o;
+}
+
+template <int OUT>
+__global__ __launch_bounds__(256) void k_trace(const rf_lens_element* __restrict__ lens,
+                                               const int* __restrict__ paths,
+                                               const int* __restrict__ wavelengths,
+                                               rf_trace_params prm, void* __restrict__ out,
+                                               void* __restrict__ inters,
+                                               const float* __restrict__ coating_lambda) {
+#pragma clang fp contract(off)
+    __shared__ float s_n[RF_LMAX];
+    const int L = prm.lens_count;
+    const int w = blockIdx.y, p = blockIdx.z;
+    const int g = prm.grid_count, R = g * g, W = prm.wavelength_count;
+    const float wavelength = (float)wavelengths[w];
+    const bool disperse = !isnan(lens[0].coefficients[0]);
+    if ((int)threadIdx.x < L) {
+        const rf_lens_element& e = lens[threadIdx.x];
+        float n = e.ior;
+        if (disperse && n > 1) n = dispersion(wavelength, e.coefficients);
+        s_n[threadIdx.x] = n;
+    }
+    __syncthreads();
+
+    const int ray_id = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = ray_id < R;
+    const long idx = ((long)p * W + w) * R + ray_id;
+    const RayOut r = trace_ray<OUT>(lens, L, prm.aperture_index, paths[2 * p], paths[2 * p + 1],
+                                    wavelength, s_n, g, prm.grid_length, prm.direction,
+                                    prm.coating_min_ior, coating_lambda, ray_id, active, inters, idx);
     if (!active) return;
 
     if (OUT == OUT_PLANES) {
         const long N = (long)prm.path_count * W * R;
         float* o = (float*)out;
-        o[idx] = pos.x;
-        o[N + idx] = pos.y;
-        o[2 * N + idx] = apt_x;
-        o[3 * N + idx] = apt_y;
-        o[4 * N + idx] = rrel;
-        o[5 * N + idx] = refl;
+        o[idx] = r.pos.x;
+        o[N + idx] = r.pos.y;
+        o[2 * N + idx] = r.apt_x;
+        o[3 * N + idx] = r.apt_y;
+        o[4 * N + idx] = r.rrel;
+        o[5 * N + idx] = r.refl;
     } else {
         float4* o = (float4*)out + 3 * idx;
-        o[0] = make_float4(pos.x, pos.y, pos.z, 0.0f);
-        o[1] = make_float4(dir.x, dir.y, dir.z, 0.0f);
-        o[2] = make_float4(apt_x, apt_y, rrel, refl);
+        o[0] = make_float4(r.pos.x, r.pos.y, r.pos.z, 0.0f);
+        o[1] = make_float4(r.dir.x, r.dir.y, r.dir.z, 0.0f);
+        o[2] = make_float4(r.apt_x, r.apt_y, r.rrel, r.refl);
     }
 }
 
@@ -287,6 +315,92 @@ static int launch_trace(const rf_lens_element* lens, const int32_t* paths,
     return RF_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Batched trace: a persistent grid drains a queue of chunks.  A chunk is
+// 256 * RPL rays of one (segment, wavelength); the chunks of segment s are
+// chunk_offsets[s] .. chunk_offsets[s + 1] - 1, wavelength-major.  Lane t of a
+// workgroup traces rays t, t + 256, ... of the chunk, so each pass over the
+// chunk's rays is coalesced and every pass keeps the path, wavelength and lens
+// wave-uniform.
+
+static constexpr int BATCH_THREADS = 256;
+
+__global__ void k_queue_reset(uint64_t* __restrict__ head) { *head = 0; }
+
+__device__ __forceinline__ int find_segment(const uint64_t* __restrict__ offsets, int n,
+                                            uint64_t c) {
+    // largest s with offsets[s] <= c
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (offsets[mid] <= c) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int RPL>
+__global__ __launch_bounds__(BATCH_THREADS) void k_trace_batch(
+    const rf_lens_element* __restrict__ lenses, const int* __restrict__ lens_table,
+    const int* __restrict__ paths, const int* __restrict__ wavelengths, int W,
+    float coating_min_ior, const rf_batch_segment* __restrict__ segs, int nseg,
+    const uint64_t* __restrict__ chunk_offsets, long ray_total, uint64_t* __restrict__ head,
+    float* __restrict__ planes) {
+#pragma clang fp contract(off)
+    __shared__ float s_n[RF_LMAX];
+    __shared__ uint64_t s_chunk;
+    const uint64_t total = chunk_offsets[nseg];
+    int cur_lens = -1, cur_w = -1;
+    for (;;) {
+        if (threadIdx.x == 0) s_chunk = atomicAdd((unsigned long long*)head, 1ull);
+        __syncthreads();
+        const uint64_t c = s_chunk;
+        __syncthreads();
+        if (c >= total) break;
+        const int si = find_segment(chunk_offsets, nseg, c);
+        const rf_batch_segment sg = segs[si];
+        const int g = sg.grid_count, R = g * g;
+        const int per_w = (R + BATCH_THREADS * RPL - 1) / (BATCH_THREADS * RPL);
+        const int local = (int)(c - chunk_offsets[si]);
+        const int w = local / per_w, blk = local - w * per_w;
+        const int first = lens_table[3 * sg.lens], L = lens_table[3 * sg.lens + 1];
+        const int aperture = lens_table[3 * sg.lens + 2];
+        const rf_lens_element* lens = lenses + first;
+        const float wavelength = (float)wavelengths[w];
+        if (sg.lens != cur_lens || w != cur_w) {
+            // the previous chunk's lanes are done with s_n (barrier above)
+            const bool disperse = !isnan(lens[0].coefficients[0]);
+            if ((int)threadIdx.x < L) {
+                const rf_lens_element& e = lens[threadIdx.x];
+                float n = e.ior;
+                if (disperse && n > 1) n = dispersion(wavelength, e.coefficients);
+                s_n[threadIdx.x] = n;
+            }
+            __syncthreads();
+            cur_lens = sg.lens;
+            cur_w = w;
+        }
+        const int b1 = paths[2 * sg.path], b2 = paths[2 * sg.path + 1];
+#pragma unroll
+        for (int k = 0; k < RPL; k++) {
+            const int ray_id = (blk * RPL + k) * BATCH_THREADS + threadIdx.x;
+            const bool active = ray_id < R;
+            const RayOut r = trace_ray<OUT_PLANES>(lens, L, aperture, b1, b2, wavelength, s_n, g,
+                                                   sg.grid_length, sg.direction, coating_min_ior,
+                                                   nullptr, ray_id, active, nullptr, 0);
+            if (active) {
+                const long idx = sg.ray_offset + (long)w * R + ray_id;
+                planes[idx] = r.pos.x;
+                planes[ray_total + idx] = r.pos.y;
+                planes[2 * ray_total + idx] = r.apt_x;
+                planes[3 * ray_total + idx] = r.apt_y;
+                planes[4 * ray_total + idx] = r.rrel;
+                planes[5 * ray_total + idx] = r.refl;
+            }
+        }
+    }
+}
+
 }  // namespace rf
 
 extern "C" int rf_trace(const rf_lens_element* lens, const int32_t* paths,
@@ -317,4 +431,56 @@ extern "C" int rf_trace_planes_coated(const rf_lens_element* lens, const float* 
     if (!coating_lambda) return rf::set_error(RF_ERR_INVALID, "rf_trace_planes_coated: NULL");
     return rf::launch_trace<rf::OUT_PLANES>(lens, paths, wavelengths, prm, planes, nullptr, stream,
                                             coating_lambda);
+}
+
+extern "C" size_t rf_trace_batch_workspace_size(void) { return 256; }
+
+extern "C" int rf_trace_batch(const rf_lens_element* lenses, const int32_t* lens_table,
+                              const int32_t* paths, const int32_t* wavelengths,
+                              int32_t wavelength_count, float coating_min_ior,
+                              const rf_batch* batch, const uint64_t* chunk_offsets,
+                              int32_t rays_per_lane, int32_t workgroups, void* workspace,
+                              size_t workspace_bytes, float* planes, rf_stream_t stream) {
+    using namespace rf;
+    if (!batch || !lenses || !lens_table || !paths || !wavelengths || !chunk_offsets || !planes)
+        return set_error(RF_ERR_INVALID, "rf_trace_batch: NULL argument");
+    if (batch->segment_count < 0 || (batch->segment_count > 0 && !batch->segments))
+        return set_error(RF_ERR_INVALID, "rf_trace_batch: bad segment list");
+    if (wavelength_count < 1 || wavelength_count > 65535)
+        return set_error(RF_ERR_INVALID, "rf_trace_batch: wavelength_count %d", wavelength_count);
+    if (workgroups < 1 || workgroups > (1 << 20))
+        return set_error(RF_ERR_INVALID, "rf_trace_batch: workgroups %d", workgroups);
+    if (!workspace || workspace_bytes < rf_trace_batch_workspace_size())
+        return set_error(RF_ERR_WORKSPACE, "rf_trace_batch: workspace %zu < %zu", workspace_bytes,
+                         rf_trace_batch_workspace_size());
+    if (batch->segment_count == 0) return RF_OK;
+    hipStream_t s = (hipStream_t)stream;
+    uint64_t* head = (uint64_t*)workspace;
+    hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(1), 0, s, head);
+    const dim3 grid(workgroups), block(BATCH_THREADS);
+    switch (rays_per_lane) {
+        case 1:
+            hipLaunchKernelGGL(k_trace_batch<1>, grid, block, 0, s, lenses, lens_table, paths,
+                               wavelengths, wavelength_count, coating_min_ior, batch->segments,
+                               batch->segment_count, chunk_offsets, (long)batch->ray_total, head,
+                               planes);
+            break;
+        case 2:
+            hipLaunchKernelGGL(k_trace_batch<2>, grid, block, 0, s, lenses, lens_table, paths,
+                               wavelengths, wavelength_count, coating_min_ior, batch->segments,
+                               batch->segment_count, chunk_offsets, (long)batch->ray_total, head,
+                               planes);
+            break;
+        case 4:
+            hipLaunchKernelGGL(k_trace_batch<4>, grid, block, 0, s, lenses, lens_table, paths,
+                               wavelengths, wavelength_count, coating_min_ior, batch->segments,
+                               batch->segment_count, chunk_offsets, (long)batch->ray_total, head,
+                               planes);
+            break;
+        default:
+            return set_error(RF_ERR_INVALID, "rf_trace_batch: rays_per_lane %d not in {1, 2, 4}",
+                             rays_per_lane);
+    }
+    RF_LAUNCH_CHECK("k_trace_batch");
+    return RF_OK;
 }

@@ -89,6 +89,13 @@ def ptr(t) -> ctypes.c_void_p:
     return ctypes.c_void_p(t.data_ptr())
 
 
+def ptr_at(t, offset_bytes: int) -> ctypes.c_void_p:
+    """Device address `offset_bytes` into tensor `t`."""
+    if not 0 <= offset_bytes <= t.numel() * t.element_size():
+        raise ValueError('offset outside the tensor')
+    return ctypes.c_void_p(ptr(t).value + int(offset_bytes))
+
+
 def exported_symbols() -> list[str]:
     """Names declared in include/realflare_amd.h (parsed from the header)."""
     import re
