@@ -310,6 +310,65 @@ def batched_timing(steps: int, warmup: int, n_frames: int = 64, tune_frames: int
     }
 
 
+def fit_timing(timed_steps: int = 3, fit_steps: int = 100) -> dict:
+    """Batched + differentiable configuration: 16 synthetic lenses of 8..16
+    surfaces (all 28..120 refracting-pair ghosts each), 32 light positions
+    each on a seeded spiral, a seeded grid of 64^2..128^2 rays per ghost, 8
+    wavelengths; one rf_trace_fit launch per gradient step (forward, loss
+    against traces of 1%-perturbed lenses, adjoint).  `timed_steps` steps are
+    timed on the full batch; a `fit_steps`-step Adam fit runs on the same
+    lenses with 2 light positions and 16^2 grids (loss reported)."""
+    import torch
+
+    from realflare_amd.adjoint import perturbed
+    from realflare_amd.batch import spiral_positions
+    from realflare_amd.fit import FitBatch, fit, make_instances
+    from realflare_amd.synthetic import synthetic_lens
+
+    res, sensor = (2048, 2048), (36.0, 24.0)
+    wavelengths = [int(round(400 + 300 * i / 7)) for i in range(8)]
+    models = [synthetic_lens(8 + i % 9, 300 + 17 * i, fstop=2.8) for i in range(16)]
+    positions = spiral_positions(32, seed=11)
+    inst = make_instances(models, positions, grid_range=(64, 128), sensor=sensor, seed=3)
+    target = [perturbed(i.params, i.static, 0.01, seed=40 + k) for k, i in enumerate(inst)]
+    fb = FitBatch(inst, wavelengths, res, sensor, checkpoint_every=4, rays_per_lane=4)
+    fb.set_targets(target)
+    params = [i.params for i in inst]
+    fb.step(params)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(timed_steps):
+        losses, _ = fb.step(params)
+    b.record()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / timed_steps
+    ms = a.elapsed_time(b) / timed_steps
+    rays = fb.rays
+    ghosts = sum(len(i.paths) for i in inst)
+    del fb
+    torch.cuda.empty_cache()
+
+    small = make_instances(models, positions[:2], grid_range=(16, 16), sensor=sensor, seed=3)
+    fs = FitBatch(small, wavelengths, res, sensor, checkpoint_every=4, rays_per_lane=2)
+    fs.set_targets(target)
+    _, hist = fit(fs, [i.params for i in small], fit_steps)
+    losses_fit = [h['loss'] for h in hist]
+    return {
+        'lens_instances': len(inst), 'surfaces': [len(m.lens_elements) - 1 for m in models],
+        'ghosts': ghosts, 'light_positions': len(positions), 'wavelengths': len(wavelengths),
+        'rays_per_step': rays, 'ms_per_step': ms, 'wall_ms_per_step': wall * 1e3,
+        'rays_per_s_forward_plus_adjoint': rays / (ms * 1e-3),
+        'fit': {'steps': fit_steps, 'light_positions': 2, 'grid': 16,
+                'rays_per_step': fs.rays, 'loss_first': losses_fit[0],
+                'loss_best': hist[-1]['best_loss'],
+                'loss_at': {str(k): hist[k]['best_loss'] for k in (0, 10, 25, 50, 99)
+                            if k < len(hist)},
+                'ms_per_step_mean': sum(h['gpu_ms'] for h in hist) / len(hist)},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -319,6 +378,7 @@ def main() -> int:
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-batched', action='store_true', help='skip the batched configuration')
     ap.add_argument('--batch-frames', type=int, default=64)
+    ap.add_argument('--no-fit', action='store_true', help='skip the batched fit configuration')
     ap.add_argument('--copy-calibration', action='store_true',
                     help='also run a device-to-device copy of 256 MiB (for counter scaling)')
     args = ap.parse_args()
@@ -419,6 +479,9 @@ def main() -> int:
     if args.workload == 'full' and not args.no_batched:
         line['batched'] = batched_timing(max(1, min(args.steps, 3)), 1, args.batch_frames)
         print(f"# batched: {line['batched']}", file=sys.stderr, flush=True)
+    if args.workload == 'full' and not args.no_fit:
+        line['batched_fit'] = fit_timing()
+        print(f"# batched fit: {line['batched_fit']}", file=sys.stderr, flush=True)
     if args.copy_calibration:
         src = torch.empty((64 << 20,), dtype=torch.float32, device='cuda')
         dst = torch.empty_like(src)

@@ -198,6 +198,42 @@ int rf_trace_batch(const rf_lens_element *lenses, const int32_t *lens_table,
                    int32_t rays_per_lane, int32_t workgroups, void *workspace,
                    size_t workspace_bytes, float *planes, rf_stream_t stream);
 
+/*
+ * One gradient step of a batched lens fit.  No counterpart in the reference
+ * (no gradients): every segment's rays are traced as rf_trace_planes_coated
+ * traces them, scored against targets, and differentiated, all in one
+ * persistent launch.  Segments may use different lens instances
+ * (rf_batch_segment.lens indexes lens_table); they must be ordered by
+ * instance, instance i owning segments lens_segment_offsets[i] ..
+ * lens_segment_offsets[i+1]-1 (device uint64[lens_instances + 1]).
+ *   lenses / coatings   records and coating wavelengths of all instances,
+ *                       concatenated; lens_table int32[instances][3] = first
+ *                       element, lens_count, aperture_index
+ *   chunk_offsets       uint64[segment_count + 1] (device): exclusive prefix
+ *                       sum of W * ceil(g^2 / (64 * rays_per_lane)); chunk_count
+ *                       its last value
+ *   target_*            float[ray_total] per ray (NaN: no target)
+ *   loss per ray        loss_scale * (dx^2 + dy^2 + (dr / reflectance_scale)^2),
+ *                       for rays that reach the sensor in trace and target
+ *   grad_out            float[instances][(3 + W) * max_elements + 1]: d/d radius,
+ *                       d/d centre, d/d coating (max_elements each), d/d ior
+ *                       for each wavelength, then the instance's loss
+ *   forward_out         optional float[3][ray_total]: x, y, reflectance (NaN
+ *                       where the ray does not reach the sensor)
+ * Deterministic: partials are per chunk and reduced in a fixed order.
+ */
+size_t rf_trace_fit_workspace_size(int32_t max_elements, int32_t checkpoint_every,
+                                   int32_t workgroups, uint64_t chunk_count);
+int rf_trace_fit(const rf_lens_element *lenses, const float *coatings, const int32_t *lens_table,
+                 const int32_t *paths, const int32_t *wavelengths, int32_t wavelength_count,
+                 float coating_min_ior, const rf_batch *batch, const uint64_t *chunk_offsets,
+                 uint64_t chunk_count, const uint64_t *lens_segment_offsets,
+                 int32_t lens_instances, int32_t max_elements, int32_t rays_per_lane,
+                 int32_t checkpoint_every, int32_t workgroups, const float *target_x,
+                 const float *target_y, const float *target_reflectance, float reflectance_scale,
+                 float loss_scale, void *workspace, size_t workspace_bytes, float *grad_out,
+                 float *forward_out, rf_stream_t stream);
+
 /* Settings of the rasterizer kernels (rasterizing.cl:233-663), as
  * RasterizingTask.rasterize derives them (rasterizing.py:296-421). */
 typedef struct rf_raster_params {

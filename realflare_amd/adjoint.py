@@ -226,27 +226,33 @@ class TraceSpec:
 
     def chain(self, params: LensParams, g: np.ndarray) -> tuple[torch.Tensor, ...]:
         """Per-element gradients (radius, centre, coating, n_eff[w]) -> parameters."""
-        st = self.static
-        n, L, W = st.n_model, st.n_model + 1, len(self.wavelengths)
-        g_rad, g_cen, g_coat = g[:L], g[L:2 * L], g[2 * L:3 * L]
-        g_n = g[3 * L:].reshape(W, L)
-        radii = g_rad[:n] + g_cen[:n]
-        # centre_i = sum_{k<i} d_k + r_i  ->  dL/dd_k = sum_{i>k} dL/dcentre_i
-        suffix = np.cumsum(g_cen[::-1])[::-1].copy()
-        dists = suffix[1:L].copy()
-        b = params.sellmeier_b.detach().double().cpu().numpy()
-        gb = np.zeros((n, 3))
-        if st.disperse:
-            for i in range(n):
-                if not st.has_glass[i]:
-                    continue
-                for w, lam in enumerate(self.wavelengths):
-                    _, d = dn_db(b[i], st.sellmeier_c[i], lam)
-                    gb[i] += g_n[w, i] * d
-        radii = radii.copy()
-        radii[st.aperture_index] = 0.0
-        return (torch.tensor(radii), torch.tensor(dists), torch.tensor(gb),
-                torch.tensor(g_coat[:n].copy()))
+        return chain_grads(self.static, self.wavelengths, params, g)
+
+
+def chain_grads(st: LensStatic, wavelengths, params: LensParams,
+                g: np.ndarray) -> tuple[torch.Tensor, ...]:
+    """Per-element gradients laid out [radius L][centre L][coating L][n_eff W x L]
+    -> gradients of (radii, distances, Sellmeier B, coating)."""
+    n, L, W = st.n_model, st.n_model + 1, len(wavelengths)
+    g_rad, g_cen, g_coat = g[:L], g[L:2 * L], g[2 * L:3 * L]
+    g_n = g[3 * L:3 * L + W * L].reshape(W, L)
+    radii = g_rad[:n] + g_cen[:n]
+    # centre_i = sum_{k<i} d_k + r_i  ->  dL/dd_k = sum_{i>k} dL/dcentre_i
+    suffix = np.cumsum(g_cen[::-1])[::-1].copy()
+    dists = suffix[1:L].copy()
+    b = params.sellmeier_b.detach().double().cpu().numpy()
+    gb = np.zeros((n, 3))
+    if st.disperse:
+        for i in range(n):
+            if not st.has_glass[i]:
+                continue
+            for w, lam in enumerate(wavelengths):
+                _, d = dn_db(b[i], st.sellmeier_c[i], lam)
+                gb[i] += g_n[w, i] * d
+    radii = radii.copy()
+    radii[st.aperture_index] = 0.0
+    return (torch.tensor(radii), torch.tensor(dists), torch.tensor(gb),
+            torch.tensor(g_coat[:n].copy()))
 
 
 class GhostTrace(torch.autograd.Function):

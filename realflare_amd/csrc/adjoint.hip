@@ -538,6 +538,107 @@ __device__ __forceinline__ void event_rev(const LensF& Lf, const float* s_n, flo
     acc(s_lacc, L + el, Cb);
 }
 
+// init_ray (raytracing.cl:178-215) of grid ray `ray_id`: the vertical probe
+// from the grid point onto surface 0, then one unit back along the light.
+struct RayInit {
+    f3 gpt, down;
+    Inter I0;
+    St s;
+};
+
+__device__ __forceinline__ RayInit init_ray_state(const rf_lens_element* __restrict__ lens, int g,
+                                                  float grid_length, const float* direction,
+                                                  int ray_id) {
+#pragma clang fp contract(off)
+    RayInit r;
+    const int ray_y = (int)truncf((float)ray_id / (float)g);
+    const int ray_x = ray_id - ray_y * g;
+    r.gpt = mk3((float)((double)grid_length * ((double)((float)ray_x / (float)(g - 1)) - 0.5)),
+                (float)((double)grid_length * (0.5 - (double)((float)ray_y / (float)(g - 1)))),
+                1.0f);
+    r.down = mk3(0, 0, -1.0f);
+    r.I0 = intersect_fwd(r.gpt, r.down, lens[0].radius, lens[0].center, false);
+    const f3 d0 = norm3(mk3(-direction[0], -direction[1], -direction[2]));
+    r.s.pos = sub3(r.I0.p, d0);
+    r.s.dir = d0;
+    r.s.refl = 1.0f;
+    return r;
+}
+
+// Forward pass of one ray over its E events, starting from s.  The current
+// segment's input states are kept in st[] (on return, the last segment's);
+// segment start states go to checkpoint slot = segment of the ray's record
+// (sidx in a [slot][field][N] array): every segment's when store_all, else
+// all but the last.  Returns whether the ray reaches the sensor; s is the
+// final state.
+template <int K>
+__device__ __forceinline__ bool forward_ray(const LensF& Lf, const float* s_n, const Path& P, St& s,
+                                            St (&st)[K], float* scratch, long N, long sidx,
+                                            bool active, bool store_all) {
+    bool alive = active;
+    const int nseg = (P.E + K - 1) / K;
+    for (int seg = 0; seg < nseg; seg++) {
+        const int k0 = seg * K;
+        if (active && (store_all || seg + 1 < nseg)) st_store(scratch, N, sidx, seg, s);
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+            if (k0 + j < P.E) {
+                st[j] = s;
+                if (alive)
+                    alive = event_fwd(Lf, s_n, P.L, P.stop, event_element(P, k0 + j),
+                                      event_reflects(P, k0 + j), s);
+            }
+        }
+    }
+    return alive;
+}
+
+// Reverse pass of a ray that reaches the sensor with final state `fin`:
+// (posb, dirb, reflb) is the adjoint of the final state.  Segments are walked
+// backwards and rebuilt from their checkpoints (the last one too unless its
+// states are still in st[]); the init_ray step closes the chain.
+template <int K>
+__device__ __forceinline__ void reverse_ray(const LensF& Lf, const float* s_n, float* s_lacc,
+                                            const Path& P, const RayInit& ri, const St& fin,
+                                            St (&st)[K], const float* scratch, long N, long sidx,
+                                            bool last_in_registers, f3 posb, f3 dirb, float reflb) {
+#pragma clang fp contract(off)
+    const int nseg = (P.E + K - 1) / K;
+    f3 hit = fin.pos;  // hit point of the event being reversed
+    for (int seg = nseg - 1; seg >= 0; seg--) {
+        const int k0 = seg * K;
+        if (!last_in_registers || seg + 1 < nseg) {
+            st[0] = st_load(scratch, N, sidx, seg);
+#pragma unroll
+            for (int j = 1; j < K; j++) {
+                if (k0 + j < P.E) {
+                    st[j] = st[j - 1];
+                    event_fwd(Lf, s_n, P.L, P.stop, event_element(P, k0 + j - 1),
+                              event_reflects(P, k0 + j - 1), st[j]);
+                }
+            }
+        }
+#pragma unroll
+        for (int j = K - 1; j >= 0; j--) {
+            if (k0 + j < P.E) {
+                event_rev(Lf, s_n, s_lacc, P, k0 + j, st[j], hit, posb, dirb, reflb);
+                hit = st[j].pos;
+            }
+        }
+    }
+    // init_ray: pos0 = I0.p - d0 with the vertical probe ray; d0 is constant
+    if (ri.I0.hit) {
+        float Rb = 0.0f, Cb = 0.0f;
+        f3 gpb = mk3(0, 0, 0), ddb = mk3(0, 0, 0);
+        const rf_lens_element& e0 = Lf.lens[0];
+        const Hit h0 = hit_at(ri.down, ri.I0.p, e0.radius, e0.center);
+        hit_bwd(h0, ri.gpt, ri.down, ri.I0.p, e0.radius, e0.center, posb, mk3(0, 0, 0), 0.0f, gpb,
+                ddb, Rb, Cb);
+        acc(s_lacc, 0, Rb);
+        acc(s_lacc, P.L, Cb);
+    }
+}
+
 // What one launch of k_trace_adjoint does.
 enum AdjMode {
     ADJ_FUSED = 0,    // forward, then reverse (the last segment's states stay in registers)
@@ -590,36 +691,12 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_trace_adjoint(
     P.E = P.b1 >= 0 ? L + 2 * (P.b1 - P.b2) : L;
     const int nseg = (P.E + K - 1) / K;
 
-    // init_ray (raytracing.cl:178-215)
-    const int ray_y = (int)truncf((float)(active ? ray_id : 0) / (float)g);
-    const int ray_x = (active ? ray_id : 0) - ray_y * g;
-    const f3 gpt = mk3((float)((double)prm.grid_length * ((double)((float)ray_x / (float)(g - 1)) - 0.5)),
-                       (float)((double)prm.grid_length * (0.5 - (double)((float)ray_y / (float)(g - 1)))),
-                       1.0f);
-    const f3 down = mk3(0, 0, -1.0f);
-    const Inter I0 = intersect_fwd(gpt, down, lens[0].radius, lens[0].center, false);
-    const f3 d0 = norm3(mk3(-prm.direction[0], -prm.direction[1], -prm.direction[2]));
-    St s;
-    s.pos = sub3(I0.p, d0);
-    s.dir = d0;
-    s.refl = 1.0f;
-
+    const RayInit ri = init_ray_state(lens, g, prm.grid_length, prm.direction, active ? ray_id : 0);
+    St s = ri.s;
     St st[K];
     bool alive = active;
     if (MODE != ADJ_REVERSE) {
-        for (int seg = 0; seg < nseg; seg++) {
-            const int k0 = seg * K;
-            if (active && (MODE == ADJ_FORWARD || seg + 1 < nseg)) st_store(scratch, N, idx, seg, s);
-#pragma unroll
-            for (int j = 0; j < K; j++) {
-                if (k0 + j < P.E) {
-                    st[j] = s;
-                    if (alive)
-                        alive = event_fwd(Lf, s_n, L, P.stop, event_element(P, k0 + j),
-                                          event_reflects(P, k0 + j), s);
-                }
-            }
-        }
+        alive = forward_ray<K>(Lf, s_n, P, s, st, scratch, N, idx, active, MODE == ADJ_FORWARD);
         if (fwd_out && active) {
             fwd_out[idx] = alive ? s.pos.x : NAN;
             fwd_out[N + idx] = alive ? s.pos.y : NAN;
@@ -638,42 +715,9 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_trace_adjoint(
         alive = !isnan(s.refl);
     }
     // reverse (rays that do not reach the sensor contribute nothing)
-    if (alive) {
-        f3 posb = mk3(gx[idx], gy[idx], 0.0f), dirb = mk3(0, 0, 0);
-        float reflb = gr[idx];
-        f3 hit = s.pos;  // hit point of the event being reversed
-        for (int seg = nseg - 1; seg >= 0; seg--) {
-            const int k0 = seg * K;
-            if (MODE == ADJ_REVERSE || seg + 1 < nseg) {
-                st[0] = st_load(scratch, N, idx, seg);
-#pragma unroll
-                for (int j = 1; j < K; j++) {
-                    if (k0 + j < P.E) {
-                        st[j] = st[j - 1];
-                        event_fwd(Lf, s_n, L, P.stop, event_element(P, k0 + j - 1),
-                                  event_reflects(P, k0 + j - 1), st[j]);
-                    }
-                }
-            }
-#pragma unroll
-            for (int j = K - 1; j >= 0; j--) {
-                if (k0 + j < P.E) {
-                    event_rev(Lf, s_n, s_lacc, P, k0 + j, st[j], hit, posb, dirb, reflb);
-                    hit = st[j].pos;
-                }
-            }
-        }
-        // init_ray: pos0 = I0.p - d0 with the vertical probe ray; d0 is constant
-        if (I0.hit) {
-            float Rb = 0.0f, Cb = 0.0f;
-            f3 gpb = mk3(0, 0, 0), ddb = mk3(0, 0, 0);
-            const Hit h0 = hit_at(down, I0.p, lens[0].radius, lens[0].center);
-            hit_bwd(h0, gpt, down, I0.p, lens[0].radius, lens[0].center, posb, mk3(0, 0, 0), 0.0f,
-                    gpb, ddb, Rb, Cb);
-            acc(s_lacc, 0, Rb);
-            acc(s_lacc, L, Cb);
-        }
-    }
+    if (alive)
+        reverse_ray<K>(Lf, s_n, s_lacc, P, ri, s, st, scratch, N, idx, MODE == ADJ_FUSED,
+                       mk3(gx[idx], gy[idx], 0.0f), mk3(0, 0, 0), gr[idx]);
     const long blk = ((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
     const long nblk = (long)gridDim.x * gridDim.y * gridDim.z;
     for (int i = 0; i < 4 * L; i++) {
@@ -742,6 +786,197 @@ static AdjKernel adj_kernel(int K, int mode = ADJ_FUSED) {
         case ADJ_REVERSE: return adj_kernel_k<ADJ_REVERSE>(K);
         default: return nullptr;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Batched fit step: lens instances x light positions x ghosts x wavelengths in
+// one persistent launch, forward + loss + reverse per ray.
+//
+// A chunk is 64 * rpl rays of one (segment, wavelength), chunks wavelength-
+// major within a segment (chunk_offsets as rf_trace_batch's for 64-lane
+// chunks).  One wave per workgroup pops chunks; each lane runs its rays one
+// after the other, keeping checkpoints in its own scratch column
+// ([workgroup][slot][field][64]), so scratch does not grow with the batch.
+// The loss of a ray that reaches the sensor in the trace and in the target is
+// loss_scale * (dx^2 + dy^2 + (dr / refl_scale)^2); its gradient seeds the
+// reverse pass in place.  Partials per chunk are laid out [chunk][4 LF + 1]:
+// radius, centre, coating, n_eff of the chunk's wavelength (each LF wide),
+// then the loss.
+
+struct FitArgs {
+    const rf_lens_element* lenses;
+    const float* coatings;
+    const int* lens_table;  // [instance][3]: first element, L, aperture index
+    const int* paths;
+    const int* wavelengths;
+    const rf_batch_segment* segs;
+    const uint64_t* chunk_offsets;
+    const float *tx, *ty, *tr;
+    uint64_t* head;
+    float* scratch;
+    float* partial;
+    float* fwd_out;
+    long ray_total;
+    int nseg, W, LF, rpl, slots;
+    float cmin, inv_refl_scale, loss_scale;
+};
+
+template <int K>
+__global__ __launch_bounds__(ADJ_THREADS) void k_trace_fit(FitArgs a) {
+#pragma clang fp contract(off)
+    __shared__ float s_n[RF_LMAX];
+    __shared__ uint64_t s_chunk;
+    extern __shared__ float s_lacc[];  // [4 L][ADJ_THREADS]
+    const int lane = threadIdx.x;
+    const uint64_t total = a.chunk_offsets[a.nseg];
+    float* scratch = a.scratch + (long)blockIdx.x * a.slots * ADJ_FIELDS * ADJ_THREADS;
+    const int pstride = 4 * a.LF + 1;
+    for (;;) {
+        if (lane == 0) s_chunk = atomicAdd((unsigned long long*)a.head, 1ull);
+        __syncthreads();
+        const uint64_t c = s_chunk;
+        __syncthreads();
+        if (c >= total) break;
+        int lo = 0, hi = a.nseg - 1;  // segment of chunk c
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (a.chunk_offsets[mid] <= c) lo = mid;
+            else hi = mid - 1;
+        }
+        const rf_batch_segment sg = a.segs[lo];
+        const int g = sg.grid_count, R = g * g;
+        const int per_w = (R + ADJ_THREADS * a.rpl - 1) / (ADJ_THREADS * a.rpl);
+        const int local = (int)(c - a.chunk_offsets[lo]);
+        const int w = local / per_w, blk = local - w * per_w;
+        const int first = a.lens_table[3 * sg.lens], L = a.lens_table[3 * sg.lens + 1];
+        const rf_lens_element* lens = a.lenses + first;
+        const float wavelength = (float)a.wavelengths[w];
+        const bool disperse = !isnan(lens[0].coefficients[0]);
+        if (lane < L) {
+            const rf_lens_element& e = lens[lane];
+            float n = e.ior;
+            if (disperse && n > 1) n = dispersion_f(wavelength, e.coefficients);
+            s_n[lane] = n;
+        }
+        for (int i = 0; i < 4 * L; i++) s_lacc[i * ADJ_THREADS + lane] = 0.0f;
+        __syncthreads();
+        LensF Lf{lens, a.coatings + first, a.cmin, wavelength};
+        Path P;
+        P.b1 = a.paths[2 * sg.path];
+        P.b2 = a.paths[2 * sg.path + 1];
+        P.L = L;
+        P.stop = a.lens_table[3 * sg.lens + 2];
+        P.E = P.b1 >= 0 ? L + 2 * (P.b1 - P.b2) : L;
+        float loss = 0.0f;
+        for (int k = 0; k < a.rpl; k++) {
+            const int ray_id = (blk * a.rpl + k) * ADJ_THREADS + lane;
+            const bool active = ray_id < R;
+            const long idx = sg.ray_offset + (long)w * R + (active ? ray_id : 0);
+            const RayInit ri =
+                init_ray_state(lens, g, sg.grid_length, sg.direction, active ? ray_id : 0);
+            St s = ri.s;
+            St st[K];
+            const bool alive =
+                forward_ray<K>(Lf, s_n, P, s, st, scratch, ADJ_THREADS, lane, active, false);
+            if (a.fwd_out && active) {
+                a.fwd_out[idx] = alive ? s.pos.x : NAN;
+                a.fwd_out[a.ray_total + idx] = alive ? s.pos.y : NAN;
+                a.fwd_out[2 * a.ray_total + idx] = alive ? s.refl : NAN;
+            }
+            if (!alive) continue;
+            const float xt = a.tx[idx], yt = a.ty[idx], rt = a.tr[idx];
+            if (isnan(xt) || isnan(yt) || isnan(rt)) continue;
+            const float dx = s.pos.x - xt, dy = s.pos.y - yt, dr = (s.refl - rt) * a.inv_refl_scale;
+            loss += a.loss_scale * (dx * dx + dy * dy + dr * dr);
+            const float gs = 2.0f * a.loss_scale;
+            reverse_ray<K>(Lf, s_n, s_lacc, P, ri, s, st, scratch, ADJ_THREADS, lane, true,
+                           mk3(gs * dx, gs * dy, 0.0f), mk3(0, 0, 0),
+                           gs * dr * a.inv_refl_scale);
+        }
+        float* out = a.partial + (long)c * pstride;
+        for (int b = 0; b < 4; b++) {
+            for (int e = 0; e < a.LF; e++) {
+                const float v = e < L ? wave_sum(s_lacc[(b * L + e) * ADJ_THREADS + lane]) : 0.0f;
+                if (lane == 0) out[b * a.LF + e] = v;
+            }
+        }
+        const float lv = wave_sum(loss);
+        if (lane == 0) out[4 * a.LF] = lv;
+        __syncthreads();  // s_n and s_lacc are rewritten for the next chunk
+    }
+}
+
+// Per lens instance: out[i][b LF + e] for b < 3 (radius, centre, coating),
+// out[i][(3 + w) LF + e] (n_eff at wavelength w), out[i][(3 + W) LF] (loss).
+// Instance i owns segments seg_off[i] .. seg_off[i + 1] - 1.  Lane j sums a
+// fixed subset of chunks (or of segments, for n_eff) in order; lanes combine
+// in a fixed tree.
+__global__ __launch_bounds__(RED_THREADS) void k_fit_reduce(const float* __restrict__ partial,
+                                                            const uint64_t* __restrict__ chunk_offsets,
+                                                            const uint64_t* __restrict__ seg_off,
+                                                            int LF, int W,
+                                                            float* __restrict__ out) {
+    __shared__ float s_w[RED_THREADS / 64];
+    const int t = blockIdx.x, inst = blockIdx.y;
+    const int nout = (3 + W) * LF + 1;
+    const int pstride = 4 * LF + 1;
+    const uint64_t s0 = seg_off[inst], s1 = seg_off[inst + 1];
+    float v = 0.0f;
+    if (t < 3 * LF || t == nout - 1) {
+        const int col = t == nout - 1 ? 4 * LF : t;
+        const uint64_t c0 = chunk_offsets[s0], c1 = chunk_offsets[s1];
+        for (uint64_t c = c0 + threadIdx.x; c < c1; c += RED_THREADS) v += partial[c * pstride + col];
+    } else {
+        const int w = (t - 3 * LF) / LF, e = (t - 3 * LF) % LF;
+        for (uint64_t sg = s0 + threadIdx.x; sg < s1; sg += RED_THREADS) {
+            const uint64_t c0 = chunk_offsets[sg], per_w = (chunk_offsets[sg + 1] - c0) / W;
+            for (uint64_t c = c0 + w * per_w; c < c0 + (w + 1) * per_w; c++)
+                v += partial[c * pstride + 3 * LF + e];
+        }
+    }
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float r = 0.0f;
+        for (int k = 0; k < RED_THREADS / 64; k++) r += s_w[k];
+        out[(long)inst * nout + t] = r;
+    }
+}
+
+__global__ void k_queue_reset_fit(uint64_t* __restrict__ head) { *head = 0; }
+
+typedef void (*FitKernel)(FitArgs);
+static FitKernel fit_kernel(int K) {
+    switch (K) {
+        case 1: return k_trace_fit<1>;
+        case 2: return k_trace_fit<2>;
+        case 3: return k_trace_fit<3>;
+        case 4: return k_trace_fit<4>;
+        case 6: return k_trace_fit<6>;
+        case 8: return k_trace_fit<8>;
+        default: return nullptr;
+    }
+}
+
+struct FitLayout {
+    size_t head, scratch, partial, total;
+    int slots;
+};
+
+static FitLayout fit_layout(int max_elements, int checkpoint_every, int workgroups,
+                            uint64_t chunk_count) {
+    FitLayout F;
+    F.slots = (3 * max_elements + checkpoint_every - 1) / checkpoint_every;
+    size_t o = 0;
+    F.head = o;
+    o += 256;
+    F.scratch = o;
+    o += align_up((size_t)workgroups * F.slots * ADJ_FIELDS * ADJ_THREADS * sizeof(float), 256);
+    F.partial = o;
+    o += align_up((size_t)chunk_count * (4 * max_elements + 1) * sizeof(float), 256);
+    F.total = o;
+    return F;
 }
 
 static int max_events(const int* host_paths, int P, int L) {
@@ -870,4 +1105,79 @@ extern "C" int rf_trace_backward(const rf_lens_element* lens, const float* coati
     return adj_launch(ADJ_REVERSE, lens, coating_lambda, paths, wavelengths, prm, checkpoint_every,
                       grad_pos_x, grad_pos_y, grad_reflectance, A, workspace, grad_out, nullptr,
                       stream);
+}
+
+extern "C" size_t rf_trace_fit_workspace_size(int32_t max_elements, int32_t checkpoint_every,
+                                              int32_t workgroups, uint64_t chunk_count) {
+    if (max_elements < 2 || max_elements > RF_LMAX || !fit_kernel(checkpoint_every) ||
+        workgroups < 1)
+        return 0;
+    return fit_layout(max_elements, checkpoint_every, workgroups, chunk_count).total;
+}
+
+extern "C" int rf_trace_fit(const rf_lens_element* lenses, const float* coatings,
+                            const int32_t* lens_table, const int32_t* paths,
+                            const int32_t* wavelengths, int32_t wavelength_count,
+                            float coating_min_ior, const rf_batch* batch,
+                            const uint64_t* chunk_offsets, uint64_t chunk_count,
+                            const uint64_t* lens_segment_offsets, int32_t lens_instances,
+                            int32_t max_elements, int32_t rays_per_lane, int32_t checkpoint_every,
+                            int32_t workgroups, const float* target_x, const float* target_y,
+                            const float* target_reflectance, float reflectance_scale,
+                            float loss_scale, void* workspace, size_t workspace_bytes,
+                            float* grad_out, float* forward_out, rf_stream_t stream) {
+    if (!lenses || !coatings || !lens_table || !paths || !wavelengths || !batch ||
+        !chunk_offsets || !lens_segment_offsets || !target_x || !target_y ||
+        !target_reflectance || !grad_out || !batch->segments)
+        return set_error(RF_ERR_INVALID, "rf_trace_fit: NULL argument");
+    if (batch->segment_count < 1 || lens_instances < 1 || wavelength_count < 1 ||
+        rays_per_lane < 1 || workgroups < 1 || workgroups > (1 << 20))
+        return set_error(RF_ERR_INVALID, "rf_trace_fit: empty or inconsistent batch");
+    if (max_elements < 2 || max_elements > RF_LMAX)
+        return set_error(RF_ERR_INVALID, "rf_trace_fit: max_elements %d", max_elements);
+    const FitKernel kern = fit_kernel(checkpoint_every);
+    if (!kern)
+        return set_error(RF_ERR_INVALID, "rf_trace_fit: checkpoint_every %d not in {1, 2, 3, 4, 6, 8}",
+                         checkpoint_every);
+    if (!(reflectance_scale > 0))
+        return set_error(RF_ERR_INVALID, "rf_trace_fit: reflectance_scale must be positive");
+    const FitLayout F = fit_layout(max_elements, checkpoint_every, workgroups, chunk_count);
+    if (!workspace || workspace_bytes < F.total)
+        return set_error(RF_ERR_WORKSPACE, "rf_trace_fit: workspace %zu < %zu", workspace_bytes,
+                         F.total);
+    hipStream_t s = (hipStream_t)stream;
+    char* w = (char*)workspace;
+    FitArgs a;
+    a.lenses = lenses;
+    a.coatings = coatings;
+    a.lens_table = lens_table;
+    a.paths = paths;
+    a.wavelengths = wavelengths;
+    a.segs = batch->segments;
+    a.chunk_offsets = chunk_offsets;
+    a.tx = target_x;
+    a.ty = target_y;
+    a.tr = target_reflectance;
+    a.head = (uint64_t*)(w + F.head);
+    a.scratch = (float*)(w + F.scratch);
+    a.partial = (float*)(w + F.partial);
+    a.fwd_out = forward_out;
+    a.ray_total = batch->ray_total;
+    a.nseg = batch->segment_count;
+    a.W = wavelength_count;
+    a.LF = max_elements;
+    a.rpl = rays_per_lane;
+    a.slots = F.slots;
+    a.cmin = coating_min_ior;
+    a.inv_refl_scale = 1.0f / reflectance_scale;
+    a.loss_scale = loss_scale;
+    hipLaunchKernelGGL(k_queue_reset_fit, dim3(1), dim3(1), 0, s, a.head);
+    const size_t lds = (size_t)4 * max_elements * ADJ_THREADS * sizeof(float);
+    hipLaunchKernelGGL(kern, dim3(workgroups), dim3(ADJ_THREADS), lds, s, a);
+    const int nout = (3 + wavelength_count) * max_elements + 1;
+    hipLaunchKernelGGL(k_fit_reduce, dim3(nout, lens_instances), dim3(RED_THREADS), 0, s, a.partial,
+                       chunk_offsets, lens_segment_offsets, max_elements, wavelength_count,
+                       grad_out);
+    RF_LAUNCH_CHECK("rf_trace_fit");
+    return RF_OK;
 }

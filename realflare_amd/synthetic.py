@@ -55,10 +55,32 @@ def _paraxial(radii, thicknesses, iors, y0, u0):
     return y, u, heights
 
 
+def media_pattern(n_surfaces: int) -> tuple[list, int]:
+    """Glass/air pattern after each refracting surface and the index of the
+    surface the stop sits in front of.  6 and 12 surfaces use the fixed
+    patterns above; other counts alternate singlets (glass, air) and cemented
+    doublets (glass, glass, air), end in air, and put the stop in the air gap
+    nearest the middle."""
+    if n_surfaces in _MEDIA:
+        return _MEDIA[n_surfaces], _STOP[n_surfaces]
+    if n_surfaces < 3:
+        raise ValueError('need at least 3 surfaces')
+    media, k = [], 0
+    while len(media) < n_surfaces:
+        media += [True, False] if k % 2 == 0 else [True, True, False]
+        k += 1
+    media = media[:n_surfaces]
+    media[-1] = False
+    if not media[-2]:
+        media[-2] = True
+    gaps = [i for i in range(1, n_surfaces) if not media[i - 1]]
+    stop_at = min(gaps, key=lambda i: abs(i - n_surfaces / 2))
+    return media, stop_at
+
+
 def synthetic_lens(n_surfaces: int, seed: int, fstop: float = 2.0, max_tries: int = 10000,
                    grid_margin: float = 1.15) -> LensModel:
-    media = _MEDIA[n_surfaces]
-    stop_at = _STOP[n_surfaces]
+    media, stop_at = media_pattern(n_surfaces)
     for k in range(max_tries):
         rng = np.random.default_rng(seed + k)
         radii = rng.uniform(20, 200, n_surfaces) * rng.choice([-1.0, 1.0], n_surfaces)
