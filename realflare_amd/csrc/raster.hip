@@ -619,10 +619,13 @@ __device__ __forceinline__ void barycentric_quad(int px, int py, const int* x, c
 __device__ __forceinline__ float cl_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ float cl_sqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
 
-// rasterizing.cl:95-148 (mean value coordinates of a quad), shading precision
+// rasterizing.cl:95-148 (mean value coordinates of a quad), shading precision:
+// approximate reciprocal and square root, fused multiply-adds allowed (the
+// weights only scale a fragment's colour; coverage is decided elsewhere in
+// integers).
 __device__ __forceinline__ void barycentric_fast(int px, int py, const int* x, const int* y,
                                                  float* w) {
-#pragma clang fp contract(off)
+#pragma clang fp contract(fast)
     float sx[4], sy[4], a[4], d[4], r[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
@@ -684,11 +687,11 @@ __device__ __forceinline__ bool raster_tile(unsigned bid, const Geom& G, int* tx
 }
 
 // fragment_shader (rasterizing.cl:471-506) for one covered pixel of one quad
-// step, scaled by its covered AA samples.
+// step, scaled by its covered AA samples; fused multiply-adds allowed.
 __device__ __forceinline__ float shade_fragment(int pcx, int pcy, const Setup& rec, const Attr& at,
                                                 int hits, const float* __restrict__ ghost,
                                                 const Geom& G, float inv_scale) {
-#pragma clang fp contract(off)
+#pragma clang fp contract(fast)
     float w[4];
     barycentric_fast(pcx, pcy, rec.vx, rec.vy, w);
     float uu = w[0] * at.u[0] + w[1] * at.u[1] + w[2] * at.u[2] + w[3] * at.u[3];
