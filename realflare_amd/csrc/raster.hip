@@ -335,29 +335,27 @@ __device__ __forceinline__ void load_pos(const float4* __restrict__ vtx, long vi
 // edges (double precision, widened by one sample on each side).
 struct HullRows {
     double x[4], y[4];
-    double slope[6];
-    int ia[6], ib[6], n;
+    double slope[6];  // pairs (0,1) (0,2) (0,3) (1,2) (1,3) (2,3)
+    bool valid[6];    // the pair's vertices differ in y
 };
 
+// vertex pairs, indexed by compile-time constants only so the hull stays in
+// registers
+__host__ __device__ constexpr int hull_pa(int s) { return s < 3 ? 0 : (s < 5 ? 1 : 2); }
+__host__ __device__ constexpr int hull_pb(int s) { return s < 3 ? s + 1 : (s < 5 ? s - 1 : 3); }
+
 __device__ __forceinline__ void hull_prepare(const int* vx, const int* vy, HullRows* h) {
-    int n = 0;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         h->x[i] = vx[i];
         h->y[i] = vy[i];
     }
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-#pragma unroll
-        for (int j = i + 1; j < 4; j++) {
-            if (vy[i] == vy[j]) continue;
-            h->ia[n] = i;
-            h->ib[n] = j;
-            h->slope[n] = ((double)vx[j] - vx[i]) / ((double)vy[j] - vy[i]);
-            n++;
-        }
+    for (int s = 0; s < 6; s++) {
+        const int i = hull_pa(s), j = hull_pb(s);
+        h->valid[s] = vy[i] != vy[j];
+        h->slope[s] = h->valid[s] ? ((double)vx[j] - vx[i]) / ((double)vy[j] - vy[i]) : 0.0;
     }
-    h->n = n;
 }
 
 // Tiles [*a, *b] of tile row `ty` the hull can reach (empty when *a > *b).
@@ -371,8 +369,10 @@ __device__ __forceinline__ void hull_row_span(const HullRows& h, int AA, int ty,
             xmax = fmax(xmax, h.x[i]);
         }
     }
-    for (int s = 0; s < h.n; s++) {
-        const double xa = h.x[h.ia[s]], ya = h.y[h.ia[s]], yb = h.y[h.ib[s]];
+#pragma unroll
+    for (int s = 0; s < 6; s++) {
+        if (!h.valid[s]) continue;
+        const double xa = h.x[hull_pa(s)], ya = h.y[hull_pa(s)], yb = h.y[hull_pb(s)];
 #pragma unroll
         for (int k = 0; k < 2; k++) {
             const double Y = k ? Y1 : Y0;
@@ -510,23 +510,19 @@ __global__ void k_finalize(Header* __restrict__ h, uint64_t capacity) {
 }
 
 // One thread per (quad, step): walks the tile rows of its rectangle with the
-// hull span k_setup counted with and writes its (tile, entry) pairs at its
-// offset from the scan.
-__global__ __launch_bounds__(256) void k_expand(const Setup* __restrict__ setup,
-                                                const uint32_t* __restrict__ counts,
-                                                const uint64_t* __restrict__ offsets,
-                                                const Header* __restrict__ h, Geom G,
-                                                uint32_t* __restrict__ keys,
-                                                uint32_t* __restrict__ vals) {
-    const long e = (long)blockIdx.x * 256 + threadIdx.x;
-    if (e >= G.NE || h->overflow) return;
-    if (!counts[e]) return;
-    const Setup rec = setup[e];
+// hull span k_setup counted with.  A block's 256 entries own one contiguous
+// run of the pair arrays (the offsets are an exclusive scan), so the block
+// stages its pairs in LDS and stores the run coalesced; a run larger than the
+// staging area is written directly.
+static constexpr int EXPAND_STAGE = 4096;
+
+__device__ __forceinline__ void expand_entry(const Setup& rec, const Geom& G, uint32_t e,
+                                             uint64_t off, uint32_t* __restrict__ keys,
+                                             uint32_t* __restrict__ vals) {
     const uint32_t tx0 = rec.r0 & 0xffff, ty0 = rec.r0 >> 16;
     const uint32_t tx1 = rec.r1 & 0xffff, ty1 = rec.r1 >> 16;
     HullRows hull;
     hull_prepare(rec.vx, rec.vy, &hull);
-    uint64_t off = offsets[e];
     const uint32_t frame_key =
         G.batch ? (uint32_t)loc_quad(G, rec.prim).frame * (uint32_t)(G.ntx * G.nty) : 0u;
     for (uint32_t ty = ty0; ty <= ty1; ty++) {
@@ -536,9 +532,37 @@ __global__ __launch_bounds__(256) void k_expand(const Setup* __restrict__ setup,
         b = min(b, (long)tx1);
         for (long tx = a; tx <= b; tx++) {
             keys[off] = frame_key + ty * (uint32_t)G.ntx + (uint32_t)tx;
-            vals[off] = (uint32_t)e;
+            vals[off] = e;
             off++;
         }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_expand(const Setup* __restrict__ setup,
+                                                const uint32_t* __restrict__ counts,
+                                                const uint64_t* __restrict__ offsets,
+                                                const Header* __restrict__ h, Geom G,
+                                                uint32_t* __restrict__ keys,
+                                                uint32_t* __restrict__ vals) {
+    __shared__ uint32_t s_keys[EXPAND_STAGE], s_vals[EXPAND_STAGE];
+    if (h->overflow) return;
+    const long first = (long)blockIdx.x * 256;
+    const long last = min(first + 256, G.NE) - 1;  // block's last entry
+    const long e = first + threadIdx.x;
+    const uint64_t begin = offsets[first];
+    const uint64_t end = offsets[last] + counts[last];
+    const bool staged = end - begin <= EXPAND_STAGE;  // block-uniform
+    if (e <= last && counts[e]) {
+        uint32_t* kp = staged ? s_keys : keys;  // generic pointers: LDS or global
+        uint32_t* vp = staged ? s_vals : vals;
+        expand_entry(setup[e], G, (uint32_t)e, offsets[e] - (staged ? begin : 0), kp, vp);
+    }
+    if (!staged) return;
+    __syncthreads();
+    const uint32_t n = (uint32_t)(end - begin);
+    for (uint32_t i = threadIdx.x; i < n; i += 256) {
+        keys[begin + i] = s_keys[i];
+        vals[begin + i] = s_vals[i];
     }
 }
 
