@@ -437,6 +437,21 @@ int rf_starburst(const float *spectrum, int spectrum_width, int spectrum_height,
 int rf_compose_flare_starburst(const float *flare, const float *starburst, long pixel_count,
                                float *out, rf_stream_t stream);
 
+/*
+ * Image-based light composition.  Replaces the accumulation loop of
+ * Engine.image_flare (engine.py:120-151): out (float4[height][width]) gets,
+ * for each of `count` frames (float4[count][height][width], the flare of one
+ * light sample) and each q in 0..3, weights[frame][q] (RGBA) times the frame
+ * as rendered (q = 0), flipped in rows (1), in rows and columns (2), in
+ * columns (3), in that order; accumulate = 1 adds onto out's previous
+ * contents (to compose in several calls), and the result is divided by
+ * `divisor` at the end of the call (the reference's final division by the
+ * sample count w * h, given with the last call; 1 elsewhere).
+ */
+int rf_image_light_compose(const float *frames, int32_t count, int32_t width, int32_t height,
+                           const float *weights, float divisor, int32_t accumulate, float *out,
+                           rf_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

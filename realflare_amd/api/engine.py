@@ -11,6 +11,7 @@ callbacks instead of a Qt signal.
 """
 from __future__ import annotations
 
+import ctypes
 import logging
 from collections import OrderedDict
 from typing import Callable
@@ -24,6 +25,7 @@ from .data import Project, RealflareError, RenderElement
 from .tasks.aperture import GhostApertureTask, StarburstApertureTask
 from .tasks.device import Image
 from .tasks.ghost import GhostTask
+from .tasks.image_sampling import ImageSamplingTask
 from .tasks.preprocessing import PreprocessTask
 from .tasks.rasterizing import RasterizingTask
 from .tasks.raytracing import IntersectionsTask, RaytracingTask
@@ -57,6 +59,7 @@ class Engine:
         self.intersection_task = IntersectionsTask()
         self.rasterizing_task = RasterizingTask()
         self.preprocess_task = PreprocessTask(self.raytracing_task)
+        self.image_sampling_task = ImageSamplingTask()
         self.initialized = True
 
     def elements(self) -> list[RenderElement]:
@@ -86,14 +89,73 @@ class Engine:
         else:
             path_indexes = self.preprocess_task.run(project)
         if project.flare.light.image_file_enabled:
-            raise RealflareError('image-based lights are not supported by this build')
+            return self.image_flare(project, path_indexes)
         ghost = self.ghost(project)
         rays = self.raytracing_task.run(project, path_indexes)
         return self.rasterizing_task.run(project, rays, ghost)
 
+    def image_flare(self, project: Project, path_indexes, frames_per_batch: int = 16) -> Image:
+        """engine.py:98-155.  Each non-zero sample of the top-left quadrant of
+        the sample image is a light position; its flare counts for itself and
+        its three mirror images.  The lights are rendered as frames of a
+        batch (all ghosts with the project's grid, `frames_per_batch` frames
+        per BatchRenderer) and rf_image_light_compose adds the flipped frames
+        with the samples' RGBA weights, in the reference's order."""
+        from ..batch import BatchRenderer, Tuning
+        from . import lens as api_lens
+        from .tasks.raytracing import ghost_paths, render_wavelengths
+
+        ghost = self.ghost(project)
+        sample = self.image_sampling_task.run(project)
+        dev = ghost.tensor.device
+        if project.flare.light.show_image:
+            return Image(tensor=torch.from_numpy(sample).to(dev),
+                         args=(project.flare.light, project.render.resolution))
+        height, width = sample.shape[:2]
+        half_width, half_height = int(width / 2), int(height / 2)
+        positions, weights = [], []
+        for y in range(half_height):
+            for x in range(half_width):
+                values = np.float32([sample[y, x], sample[height - y - 1, x],
+                                     sample[height - y - 1, width - x - 1],
+                                     sample[y, width - x - 1]])
+                if np.sum(values) == 0:
+                    continue
+                positions.append(((x + 0.5) / half_width - 1, 1 - (y + 0.5) / half_height))
+                weights.append(values)
+        render = project.render
+        w, h = render.resolution.width(), render.resolution.height()
+        out = torch.zeros((h, w, 4), dtype=torch.float32, device=dev)
+        lens = project.flare.lens
+        lens_model = lens.lens_model_path if isinstance(lens.lens_model_path, api_lens.LensModel) \
+            else api_lens.model_from_path(lens.lens_model_path)
+        paths = ghost_paths(lens_model, tuple(path_indexes), render.ghost_paths)
+        if not positions or not paths:
+            return Image(tensor=out, args=(project.flare.light,))
+        wavelengths = render_wavelengths(render)
+        weights_t = torch.from_numpy(np.stack(weights)).to(dev)
+        for i in range(0, len(positions), frames_per_batch):
+            chunk = positions[i:i + frames_per_batch]
+            grids = np.full((len(chunk), len(paths)), render.grid_count, np.int32)
+            br = BatchRenderer(project, lens_model, paths, wavelengths, chunk, grids,
+                               Tuning(frames_per_pass=min(4, len(chunk))),
+                               ghost_texture=ghost.tensor)
+            br.fit_capacity()
+            frames = br.render()
+            last = i + frames_per_batch >= len(positions)
+            native.call('rf_image_light_compose', native.ptr(frames), len(chunk), w, h,
+                        native.ptr_at(weights_t, i * 64),
+                        ctypes.c_float(float(width * height) if last else 1.0), int(i > 0),
+                        native.ptr(out), native.stream())
+            del br, frames
+        return Image(tensor=out, args=(project.flare.light,))
+
     def flare_starburst(self, project: Project) -> Image:
         flare = self.flare(project)
         array = flare.tensor.clone()
+        if project.flare.light.image_file_enabled:
+            logger.warning('Starburst is not yet supported for image based flares.')
+            return Image(tensor=array, args=flare.args)
         starburst = self.starburst(project)
         array += flare.tensor + starburst.tensor
         return Image(tensor=array, args=(flare.args, starburst.args))

@@ -188,7 +188,7 @@ class BatchRenderer:
 
     def __init__(self, project: Project, lens_model: LensModel, paths, wavelengths, positions,
                  grids: np.ndarray, tuning: Tuning | None = None, pair_capacity: int | None = None,
-                 device=None) -> None:
+                 device=None, ghost_texture: torch.Tensor | None = None) -> None:
         self.device = dev = device or torch.device('cuda', torch.cuda.current_device())
         self.project = project
         self.tuning = tuning or Tuning()
@@ -212,16 +212,23 @@ class BatchRenderer:
         self.coating_min_ior = lens.coating_min_ior
         self.whole = self._desc(0, self.F)
 
-        # ghost texture (shared by every frame) and light spectrum
+        # ghost texture (shared by every frame; GhostTask's when given) and
+        # light spectrum
         gres = (render.ghost.resolution.width(), render.ghost.resolution.height())
-        self.ghost_tex = torch.empty((gres[1], gres[0]), dtype=torch.float32, device=dev)
-        ap = torch.empty_like(self.ghost_tex)
-        ws = torch.empty((_size('rf_ghost_workspace_size')(*gres),), dtype=torch.uint8, device=dev)
-        native.call('rf_aperture', ctypes.byref(aperture_params(project.flare.ghost_aperture, gres)),
-                    native.ptr(ap), native.stream())
-        native.call('rf_ghost', native.ptr(ap), gres[0], gres[1],
-                    ctypes.c_float(project.flare.ghost.fstop), native.ptr(ws),
-                    ctypes.c_size_t(ws.numel()), native.ptr(self.ghost_tex), native.stream())
+        if ghost_texture is not None:
+            self.ghost_tex = ghost_texture.contiguous()
+            gres = (self.ghost_tex.shape[1], self.ghost_tex.shape[0])
+        else:
+            self.ghost_tex = torch.empty((gres[1], gres[0]), dtype=torch.float32, device=dev)
+            ap = torch.empty_like(self.ghost_tex)
+            ws = torch.empty((_size('rf_ghost_workspace_size')(*gres),), dtype=torch.uint8,
+                             device=dev)
+            native.call('rf_aperture',
+                        ctypes.byref(aperture_params(project.flare.ghost_aperture, gres)),
+                        native.ptr(ap), native.stream())
+            native.call('rf_ghost', native.ptr(ap), gres[0], gres[1],
+                        ctypes.c_float(project.flare.ghost.fstop), native.ptr(ws),
+                        ctypes.c_size_t(ws.numel()), native.ptr(self.ghost_tex), native.stream())
         self.spectrum = torch.from_numpy(light_spectrum()).to(dev)
         self.prm = raster_params(0, self.W, 2, render.grid_length, self.resolution, render.bin_size,
                                  render.anti_aliasing, render.wavelength_sub_count, self.tr.sensor,

@@ -152,3 +152,56 @@ extern "C" int rf_compose_flare_starburst(const float* flare, const float* starb
     RF_LAUNCH_CHECK("rf_compose_flare_starburst");
     return RF_OK;
 }
+
+namespace rf {
+// Engine.image_flare (engine.py:98-155): every light sample's flare is added
+// four times, as rendered and mirrored (rows; rows and columns; columns),
+// weighted by the RGBA values of the four image samples it stands for, in
+// sample order, into the running image, which the last call divides by the
+// sample count (sample width * sample height).
+__global__ __launch_bounds__(256) void k_image_light(const float4* __restrict__ frames, int count,
+                                                     int width, int height,
+                                                     const float4* __restrict__ weights,
+                                                     float divisor, int accumulate,
+                                                     float4* __restrict__ out) {
+#pragma clang fp contract(off)
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long)width * height) return;
+    const int y = (int)(i / width), x = (int)(i % width);
+    const int yf = height - 1 - y, xf = width - 1 - x;
+    const long n = (long)width * height;
+    const long src[4] = {(long)y * width + x, (long)yf * width + x, (long)yf * width + xf,
+                         (long)y * width + xf};
+    float4 acc = make_float4(0, 0, 0, 0);
+    for (int f = 0; f < count; f++) {
+        const float4* fr = frames + f * n;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const float4 v = weights[4 * f + q], p = fr[src[q]];
+            acc = make_float4(acc.x + v.x * p.x, acc.y + v.y * p.y, acc.z + v.z * p.z,
+                              acc.w + v.w * p.w);
+        }
+    }
+    if (accumulate) {
+        const float4 o = out[i];
+        acc = make_float4(o.x + acc.x, o.y + acc.y, o.z + acc.z, o.w + acc.w);
+    }
+    if (divisor != 1.0f)
+        acc = make_float4(acc.x / divisor, acc.y / divisor, acc.z / divisor, acc.w / divisor);
+    out[i] = acc;
+}
+}  // namespace rf
+
+extern "C" int rf_image_light_compose(const float* frames, int32_t count, int32_t width,
+                                      int32_t height, const float* weights, float divisor,
+                                      int32_t accumulate, float* out, rf_stream_t stream) {
+    using namespace rf;
+    if (!out || width < 1 || height < 1 || count < 0 || (count > 0 && (!frames || !weights)))
+        return set_error(RF_ERR_INVALID, "rf_image_light_compose: bad arguments");
+    const long n = (long)width * height;
+    hipLaunchKernelGGL(k_image_light, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, (const float4*)frames, count, width, height,
+                       (const float4*)weights, divisor, accumulate, (float4*)out);
+    RF_LAUNCH_CHECK("rf_image_light_compose");
+    return RF_OK;
+}
