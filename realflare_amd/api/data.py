@@ -340,6 +340,11 @@ def _resolve_hints(cls) -> dict:
     return hints
 
 
+def field_types(cls) -> dict:
+    """Field name -> resolved type of a schema dataclass."""
+    return dict(_resolve_hints(cls))
+
+
 def cast(tp, data):
     """Build ``tp`` from JSON data.
 

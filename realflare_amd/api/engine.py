@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import ctypes
 import logging
+import os
 from collections import OrderedDict
 from typing import Callable
 
@@ -21,7 +22,9 @@ import torch
 
 from .. import native
 from ..utils.timing import timer
+from . import storage
 from .data import Project, RealflareError, RenderElement
+from .output import write_array
 from .tasks.aperture import GhostApertureTask, StarburstApertureTask
 from .tasks.device import Image
 from .tasks.ghost import GhostTask
@@ -176,6 +179,7 @@ class Engine:
                     if torch.cuda.is_available():
                         torch.cuda.synchronize()
                     self.emit_image(image, element)
+                    self.write_image(image, element, project)
         except RealflareError as e:
             logger.error(e)
         except (native.NativeError, torch.cuda.OutOfMemoryError) as e:
@@ -188,6 +192,23 @@ class Engine:
             logger.exception(e)
             return False
         return True
+
+    def write_image(self, image, element: RenderElement, project: Project) -> None:
+        """engine.py:239-267: write the output element, the flare and the
+        starburst to separate files when split_files is set."""
+        out = project.output
+        if not out.write or element != out.element:
+            return
+        filename = storage.parse_output_path(out.path, out.frame)
+        if element == RenderElement.FLARE_STARBURST and out.split_files:
+            words = os.path.basename(filename).split('.')
+            for part, renderer in (('flare', self.flare), ('starburst', self.starburst)):
+                w = list(words)
+                w.insert(1, part)
+                path = os.path.join(os.path.dirname(filename), '.'.join(w))
+                write_array(renderer(project).array, path, out.colorspace)
+        else:
+            write_array(image.array, filename, out.colorspace)
 
     def emit_image(self, image, element: RenderElement) -> None:
         _hash = hash(image)

@@ -1,0 +1,48 @@
+"""The command line end to end: a project file, an animation file and frames
+1..2 render the output element to EXR files equal to direct Engine renders of
+the animated projects (cli/app.py:44-102, engine.py:239-291)."""
+import json
+
+import numpy as np
+import pytest
+
+from realflare_amd.api.data import Point, RenderElement, Size, basic
+from realflare_amd.api.engine import Engine
+from realflare_amd.api.output import read_exr
+from realflare_amd.cli import main
+from realflare_amd.synthetic import workload
+
+pytestmark = pytest.mark.gpu
+
+
+def test_cli_renders_animated_frames(tmp_path):
+    wl = workload('small')
+    p = wl.project
+    p.render.grid_count = 24
+    p.render.resolution = Size(96, 64)
+    p.render.ghost.resolution = Size(64, 64)
+    p.render.wavelengths = tuple(wl.wavelengths)
+    p.render.ghost_paths = 'all'
+    model = tmp_path / 'lens.json'
+    model.write_text(json.dumps(basic(wl.lens_model)))
+    p.flare.lens.lens_model_path = str(model)
+    (tmp_path / 'project.json').write_text(json.dumps(basic(p)))
+    positions = [[0.1, 0.2], [-0.3, 0.1]]
+    (tmp_path / 'anim.json').write_text(json.dumps({'flare': {'light': {'position': positions}}}))
+    out = str(tmp_path / 'render' / 'flare.$F4.exr')
+    rc = main(['--project', str(tmp_path / 'project.json'), '--animation',
+               str(tmp_path / 'anim.json'), '--output', out, '--element', 'FLARE',
+               '--frame-start', '1', '--frame-end', '2'])
+    assert rc == 0
+    for i, (x, y) in enumerate(positions):
+        p.flare.light.position = Point(x, y)
+        engine = Engine()
+        engine.set_elements([RenderElement.FLARE])
+        got = {}
+        engine.image_rendered.append(lambda img, el: got.__setitem__(el, img))
+        assert engine.render(p)
+        ref = got[RenderElement.FLARE].array
+        exr = read_exr(str(tmp_path / 'render' / f'flare.{i + 1:04d}.exr'))
+        for k, c in enumerate('RGB'):
+            np.testing.assert_array_equal(exr[c], ref[..., k])
+        assert np.abs(ref).sum() > 0

@@ -1,0 +1,62 @@
+"""CLI animation and output writing (cli/app.py:22-102, engine.py:239-291),
+host side: per-frame animation values (the reference's CLI animation
+fixture), frame-number substitution in output paths, and the EXR writer
+read back (header fields and every float bit preserved)."""
+import json
+import os
+
+import numpy as np
+
+from realflare_amd.api.data import Point, Project
+from realflare_amd.api.output import read_exr, write_array, write_exr
+from realflare_amd.api.storage import parse_output_path
+from realflare_amd.cli import apply_animation, argument_parser
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def test_animation_fixture_sets_light_positions_per_frame():
+    with open(os.path.join(HERE, 'golden', 'cli_animation.json')) as f:
+        anim = json.load(f)
+    p = Project()
+    seen = []
+    for i in range(4):
+        apply_animation(p, anim, i)
+        seen.append(p.flare.light.position)
+    assert all(isinstance(v, Point) for v in seen)
+    assert [(v.x(), v.y()) for v in seen] == [(0.5, 0.5), (1.0, 1.0), (1.5, 1.5), (1.5, 1.5)]
+
+
+def test_output_path_frames(tmp_path):
+    assert parse_output_path(str(tmp_path / 'out.$F4.exr'), 7).endswith('out.0007.exr')
+    assert parse_output_path(str(tmp_path / 'out.%04d.exr'), 12).endswith('out.0012.exr')
+
+
+def test_exr_round_trip(tmp_path):
+    rng = np.random.default_rng(1)
+    chans = {c: rng.standard_normal((5, 7)).astype(np.float32) for c in 'BGR'}
+    chans['R'][0, 0] = np.inf
+    path = str(tmp_path / 'a.exr')
+    write_exr(path, chans)
+    back = read_exr(path)
+    assert sorted(back) == ['B', 'G', 'R']
+    for c in 'BGR':
+        np.testing.assert_array_equal(back[c].view(np.uint32), chans[c].view(np.uint32))
+    with open(path, 'rb') as f:
+        head = f.read(8)
+    assert head == (20000630).to_bytes(4, 'little') + (2).to_bytes(4, 'little')
+
+
+def test_write_array_exr_keeps_rgb(tmp_path):
+    a = np.random.default_rng(2).random((6, 4, 4)).astype(np.float32)
+    path = str(tmp_path / 'sub' / 'img.0001.exr')
+    write_array(a, path, 'ACES - ACEScg')
+    back = read_exr(path)
+    for k, c in enumerate('RGB'):
+        np.testing.assert_array_equal(back[c], a[..., k])
+
+
+def test_argument_parser_defaults():
+    args = argument_parser().parse_args(['--project', 'p.json'])
+    assert args.frame_start == 1 and args.frame_end == 1
+    assert args.colorspace == 'ACES - ACEScg'
