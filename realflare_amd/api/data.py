@@ -327,11 +327,13 @@ class Project:
 
 def _resolve_hints(cls) -> dict:
     hints = {}
-    localns = dict(vars(cls))
+    localns = {}
     for klass in (Project, Flare, Render, Aperture, LensModel, Diagram, Output):
         localns.setdefault(klass.__name__, klass)
         localns.update({k: v for k, v in vars(klass).items() if isinstance(v, type)})
     localns.update({'Size': Size, 'Point': Point, 'RenderElement': RenderElement})
+    # a class's own nested types win: Flare.Ghost and Render.Ghost share a name
+    localns.update({k: v for k, v in vars(cls).items() if isinstance(v, type)})
     for f in dataclasses.fields(cls):
         tp = f.type
         if isinstance(tp, str):

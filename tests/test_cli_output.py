@@ -7,7 +7,7 @@ import os
 
 import numpy as np
 
-from realflare_amd.api.data import Point, Project
+from realflare_amd.api.data import Flare, Point, Project, Render, Size, basic, cast
 from realflare_amd.api.output import read_exr, write_array, write_exr
 from realflare_amd.api.storage import parse_output_path
 from realflare_amd.cli import apply_animation, argument_parser
@@ -60,3 +60,15 @@ def test_argument_parser_defaults():
     args = argument_parser().parse_args(['--project', 'p.json'])
     assert args.frame_start == 1 and args.frame_end == 1
     assert args.colorspace == 'ACES - ACEScg'
+
+
+def test_project_json_round_trip_keeps_nested_types():
+    # Flare.Ghost (fstop) and Render.Ghost (resolution) share a class name
+    p = Project()
+    p.flare.ghost.fstop = 5.6
+    p.render.ghost.resolution = Size(96, 48)
+    q = cast(Project, json.loads(json.dumps(basic(p))))
+    assert type(q.flare.ghost) is Flare.Ghost and q.flare.ghost.fstop == 5.6
+    assert type(q.render.ghost) is Render.Ghost
+    assert q.render.ghost.resolution == Size(96, 48)
+    assert basic(q) == basic(p)
