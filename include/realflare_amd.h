@@ -452,6 +452,33 @@ int rf_image_light_compose(const float *frames, int32_t count, int32_t width, in
                            const float *weights, float divisor, int32_t accumulate, float *out,
                            rf_stream_t stream);
 
+/*
+ * Lens diagram, first pass.  Replaces the `lenses` kernel
+ * (diagram.cl:21-68) as DiagramTask.lenses launches it (diagram.py:113-158):
+ * writes every pixel of image (float4[height][width]) with the optical axis
+ * in red, flat elements (radius 0) as a column (the stop and the last
+ * element green, others blue) and spherical elements as a half circle in
+ * blue; scale is pixels per lens unit ((width - 10) / total element distance,
+ * diagram.py:44-56).
+ */
+int rf_diagram_lenses(const rf_lens_element *lens, int32_t lens_count, int32_t aperture_index,
+                      float scale, int32_t width, int32_t height, float *image,
+                      rf_stream_t stream);
+
+/*
+ * Lens diagram, second pass.  Replaces the `intersections` kernel
+ * (diagram.cl:71-138) as DiagramTask.intersections launches it
+ * (diagram.py:72-111): draws over image the surface normals (overwriting the
+ * pixel with (0.25, 0, 0, 0)) and the ray segments (adding 0.25 to R, G, B)
+ * of ray_count rays.  intersections is Intersection[ray_count][records_per_ray]
+ * (48-byte records as rf_trace_intersections writes them; the reference
+ * passes the middle grid column of one ghost, diagram.py:58-70); a ray's
+ * records stop at the first record with hit == 0.
+ */
+int rf_diagram_intersections(const void *intersections, int32_t records_per_ray,
+                             int32_t ray_count, float scale, int32_t width, int32_t height,
+                             float *image, rf_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

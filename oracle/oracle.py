@@ -12,6 +12,8 @@ the render path in ``realflare_amd`` never does.
   (``realflare/api/tasks/starburst.py:45-81``) and the ghost texture's
   fractional Fourier transform (``realflare/api/tasks/ghost.py:13-36``,
   ``realflare/utils/frft.py:8-62``).
+* The lens diagram kernels (``realflare/api/tasks/cl/diagram.cl``) are
+  restated in float32 numpy, vectorized over pixels.
 
 Pinning: ``tests/test_oracle.py`` and ``tests/test_host.py`` check these
 functions against the reference's test vectors, closed forms, invariants,
@@ -333,6 +335,92 @@ def ghost(aperture: np.ndarray, fstop: float, resolution) -> np.ndarray:
     array = np.float32(np.abs(spectrum))
     array *= np.sqrt(resolution[0] * resolution[1])
     return array
+
+
+def _trunc_eq(a, b):
+    """(int) a == (int) b for values inside the int range (C truncation)."""
+    return np.trunc(a) == np.trunc(b)
+
+
+def diagram_lenses(radius, height, center, aperture_index: int, scale: float, width: int,
+                   height_px: int) -> np.ndarray:
+    """diagram.cl:21-68 (`lenses`), in float32 like the kernel, vectorized over
+    pixels: axis row red, flat elements a column (stop and last element green,
+    others blue, and the element loop stops there), spherical elements the
+    ring trunc(|r|) == trunc(distance) on one half, within the element height.
+    ``pow(v, 2)`` is written v * v (the correctly rounded square)."""
+    f32 = np.float32
+    s = f32(scale)
+    x = np.broadcast_to(np.arange(width).astype(f32)[None, :], (height_px, width))
+    y = np.broadcast_to((height_px // 2 - np.arange(height_px)).astype(f32)[:, None],
+                        (height_px, width))
+    rgba = np.zeros((height_px, width, 4), f32)
+    rgba[..., 0] = np.where(_trunc_eq(y, f32(0)), f32(1), f32(0))
+    done = np.zeros((height_px, width), bool)
+    n = len(radius)
+    for i in range(n):
+        r = f32(radius[i]) * s
+        h = f32(height[i]) * s
+        c = f32(center[i]) * s
+        flat = ~done & (r == 0) & (np.abs(y) < h) & _trunc_eq(x, c)
+        rgba[flat, 1 if (i == aperture_index or i == n - 1) else 2] = 1
+        dx = x - c
+        r_circle = np.sqrt(dx * dx + y * y)
+        half = (x < c) if r > 0 else (x > c)
+        ring = ~done & ~flat & half & _trunc_eq(np.abs(r), r_circle) & (np.abs(y) < h)
+        rgba[ring, 2] = 1
+        done |= flat
+    return rgba
+
+
+def diagram_is_line(px, py, x0, y0, x1, y1, width: float = 1.0):
+    """diagram.cl:1-19 in float32: strictly between the end caps and within
+    width / 2 of the line."""
+    f32 = np.float32
+    with np.errstate(divide='ignore', invalid='ignore', over='ignore'):
+        x0, y0, x1, y1 = f32(x0), f32(y0), f32(x1), f32(y1)
+        vx, vy = x1 - x0, y1 - y0
+        ax, ay = px - x0, py - y0
+        bx, by = px - x1, py - y1
+        between = ((vx * ax + vy * ay) > 0) & ((vx * bx + vy * by) < 0)
+        d = np.abs(vx * ay - ax * vy) / np.sqrt(vx * vx + vy * vy)
+        return between & (d <= f32(width) / f32(2))
+
+
+def diagram_intersections(image: np.ndarray, pos, normal, hit, records_per_ray: int,
+                          ray_count: int, scale: float) -> np.ndarray:
+    """diagram.cl:71-138 (`intersections`) over a contiguous slice of records
+    [ray][record] (fields pos (N, 3), normal (N, 3), hit (N,)): for each ray in
+    order, for each record whose successor is a hit, a normal line from the
+    point (-z, -y) * scale overwrites the pixel with (0.25, 0, 0, 0) and, unless
+    the successor lies at x == 0, a ray line to the successor adds 0.25 to
+    R, G, B.  The successor of a ray's last record is the next ray's first;
+    past the slice there is none (the reference reads beyond its buffer)."""
+    f32 = np.float32
+    s = f32(scale)
+    h, w = image.shape[:2]
+    out = np.array(image, f32, copy=True)
+    px = np.broadcast_to(np.arange(w).astype(f32)[None, :], (h, w))
+    py = np.broadcast_to((h // 2 - np.arange(h)).astype(f32)[:, None], (h, w))
+    pos = np.asarray(pos, f32)
+    normal = np.asarray(normal, f32)
+    total = ray_count * records_per_ray
+    for ray in range(ray_count):
+        for k in range(records_per_ray):
+            i1 = ray * records_per_ray + k
+            if i1 + 1 >= total or not hit[i1 + 1]:
+                break
+            p0x, p0y = (pos[i1, 2] * s) * f32(-1), (pos[i1, 1] * s) * f32(-1)
+            nx, ny = normal[i1, 2] * s, normal[i1, 1] * s
+            pnx, pny = p0x - nx * f32(1), p0y - ny * f32(1)
+            m = diagram_is_line(px, py, p0x, p0y, pnx, pny)
+            out[m] = (0.25, 0, 0, 0)
+            p1x, p1y = (pos[i1 + 1, 2] * s) * f32(-1), (pos[i1 + 1, 1] * s) * f32(-1)
+            if p1x == 0:
+                break
+            m = diagram_is_line(px, py, p0x, p0y, p1x, p1y)
+            out[m, :3] += f32(0.25)
+    return out
 
 
 def to_reference_rays(rays: np.ndarray) -> np.ndarray:

@@ -31,6 +31,7 @@ from .tasks.ghost import GhostTask
 from .tasks.image_sampling import ImageSamplingTask
 from .tasks.preprocessing import PreprocessTask
 from .tasks.rasterizing import RasterizingTask
+from .tasks.diagram import DiagramTask
 from .tasks.raytracing import IntersectionsTask, RaytracingTask
 from .tasks.starburst import StarburstTask
 
@@ -60,6 +61,7 @@ class Engine:
         self.starburst_task = StarburstTask()
         self.raytracing_task = RaytracingTask()
         self.intersection_task = IntersectionsTask()
+        self.diagram_task = DiagramTask()
         self.rasterizing_task = RasterizingTask()
         self.preprocess_task = PreprocessTask(self.raytracing_task)
         self.image_sampling_task = ImageSamplingTask()
@@ -164,8 +166,10 @@ class Engine:
         return Image(tensor=array, args=(flare.args, starburst.args))
 
     def diagram(self, project: Project):
+        """engine.py:189-193."""
         path_indexes = (project.diagram.debug_ghost,)
-        return self.intersection_task.run(project, path_indexes)
+        intersections = self.intersection_task.run(project, path_indexes)
+        return self.diagram_task.run(project, intersections)
 
     @timer
     def render(self, project: Project) -> bool:

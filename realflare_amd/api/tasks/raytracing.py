@@ -190,15 +190,31 @@ class RaytracingTask:
         )
 
 
+class IntersectionBuffer(Buffer):
+    """Intersection records kept on the device: ``tensor`` is float32
+    (P, W, g, g, 3L-1, 12), the 48-byte records rf_trace_intersections
+    writes; ``array`` is the host copy as ``intersection_dtype`` records of
+    shape (P, W, g, g, 3L-1), the reference's buffer layout
+    (raytracing.py:351-355)."""
+
+    @property
+    def array(self) -> np.ndarray:
+        if self._array is None:
+            t = self._tensor
+            self._array = t.cpu().numpy().reshape(-1).view(intersection_dtype).reshape(t.shape[:-1])
+        return self._array
+
+
 class IntersectionsTask(RaytracingTask):
     """raytracing.py:250-380: one wavelength, light at (0, diagram.light_position),
     every surface hit recorded."""
 
     @timer
-    def run(self, project: Project, path_indexes: tuple[int, ...]):
+    def run(self, project: Project, path_indexes: tuple[int, ...]) -> IntersectionBuffer | None:
         lens = project.flare.lens
         sensor_size = (lens.sensor_size.width(), lens.sensor_size.height())
-        lens_model = api_lens.model_from_path(lens.lens_model_path)
+        lens_model = api_lens.model_from_path(lens.lens_model_path) \
+            if not isinstance(lens.lens_model_path, LensModel) else lens.lens_model_path
         diagram = project.diagram
         paths = ghost_paths(lens_model, tuple(path_indexes), project.render.ghost_paths)
         inputs = self.prepare(
@@ -210,6 +226,14 @@ class IntersectionsTask(RaytracingTask):
         )
         if inputs is None:
             return None
-        _, inters = trace_records(**inputs, intersections=True)
-        g = diagram.grid_count
-        return Buffer(array=inters.reshape(len(paths), 1, g, g, -1))
+        prm = inputs['prm']
+        P, W, g = prm.path_count, prm.wavelength_count, prm.grid_count
+        n = 3 * prm.lens_count - 1
+        rays = torch.empty((P * W * g * g * 12,), dtype=torch.float32, device=self.device)
+        # records past a ray's last hit must read hit == 0
+        inters = torch.zeros((P, W, g, g, n, 12), dtype=torch.float32, device=self.device)
+        if P and W and g:
+            native.call('rf_trace_intersections', native.ptr(inputs['lens']),
+                        native.ptr(inputs['paths']), native.ptr(inputs['wavelengths']),
+                        ctypes.byref(prm), native.ptr(rays), native.ptr(inters), native.stream())
+        return IntersectionBuffer(tensor=inters)
