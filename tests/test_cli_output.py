@@ -72,3 +72,47 @@ def test_project_json_round_trip_keeps_nested_types():
     assert type(q.render.ghost) is Render.Ghost
     assert q.render.ghost.resolution == Size(96, 48)
     assert basic(q) == basic(p)
+
+
+def test_exr_layout_follows_the_file_format(tmp_path):
+    # OpenEXR file layout (single-part scanline, version 2, no flags): magic,
+    # version, attributes "name\0type\0<i32 size><value>" ending in "\0", the
+    # offset table (one u64 per scanline block), then per block <i32 y>
+    # <i32 size> and the channels' rows in alphabetical channel order.
+    import struct
+    chans = {c: np.full((3, 5), v, np.float32) for c, v in (('R', 1.0), ('G', 2.0), ('B', 3.0))}
+    path = str(tmp_path / 'layout.exr')
+    write_exr(path, chans)
+    with open(path, 'rb') as f:
+        data = f.read()
+    assert struct.unpack_from('<ii', data, 0) == (20000630, 2)
+    pos, attrs = 8, {}
+    while data[pos] != 0:
+        name_end = data.index(b'\0', pos)
+        type_end = data.index(b'\0', name_end + 1)
+        size = struct.unpack_from('<i', data, type_end + 1)[0]
+        attrs[data[pos:name_end].decode()] = (data[name_end + 1:type_end].decode(),
+                                              data[type_end + 5:type_end + 5 + size])
+        pos = type_end + 5 + size
+    pos += 1
+    required = {'channels': 'chlist', 'compression': 'compression', 'dataWindow': 'box2i',
+                'displayWindow': 'box2i', 'lineOrder': 'lineOrder', 'pixelAspectRatio': 'float',
+                'screenWindowCenter': 'v2f', 'screenWindowWidth': 'float'}
+    assert {k: v[0] for k, v in attrs.items()} == required
+    assert attrs['compression'][1] == b'\0' and attrs['lineOrder'][1] == b'\0'
+    assert struct.unpack('<iiii', attrs['dataWindow'][1]) == (0, 0, 4, 2)
+    ch = attrs['channels'][1]
+    names, q = [], 0
+    while ch[q] != 0:
+        e = ch.index(b'\0', q)
+        names.append(ch[q:e].decode())
+        ptype, plinear, xs, ys = struct.unpack_from('<iB3xii', ch, e + 1)
+        assert (ptype, plinear, xs, ys) == (2, 0, 1, 1)  # FLOAT, 1x1 sampling
+        q = e + 1 + 16
+    assert names == ['B', 'G', 'R']
+    offsets = struct.unpack_from('<3Q', data, pos)
+    for y, off in enumerate(offsets):
+        assert struct.unpack_from('<ii', data, off) == (y, 3 * 5 * 4)
+        row = np.frombuffer(data, '<f4', 15, off + 8)
+        np.testing.assert_array_equal(row, [3.0] * 5 + [2.0] * 5 + [1.0] * 5)
+    assert offsets[-1] + 8 + 60 == len(data)
