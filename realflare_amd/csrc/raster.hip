@@ -119,11 +119,26 @@ struct Header {
     uint32_t pad_;
 };
 
+// Setup.flags bit 5: every vertex coordinate lies within +-SMALL_COORD
+// samples, so the coverage test may use 24-bit multiplies (intersect_quad_small).
+static constexpr int FLAG_SMALL = 32;
+static constexpr int SMALL_COORD = 1 << 21;
+
 struct alignas(16) Setup {
     int vx[4], vy[4];
-    int flags;  // bits 0..4: top-left of edges 01,12,23,30,20
+    int flags;  // bits 0..4: top-left of edges 01,12,23,30,20; bit 5: FLAG_SMALL
     uint32_t r0, r1;  // tile rect (tx0 | ty0 << 16), (tx1 | ty1 << 16)
     uint32_t prim;
+};
+
+// Edge rows of one (quad, step) for the coverage test of rasterizing.cl:55-93:
+// for the edges 01, 12, 23, 30, 20 (vertex i to j) the edge function at
+// sample (px, py) is (x_i - x_j)(py - y_i) - (y_i - y_j)(px - x_i) =
+// a py + nb px + (-a y_i - nb x_i), with a = x_i - x_j, nb = y_j - y_i, and the
+// top-left bias (0, or -1 off a top-left edge) added into c.  All mod 2^32.
+struct alignas(16) Edges {
+    int a[5], nb[5], c[5];
+    int flags;  // as Setup.flags
 };
 
 // Shading inputs of one (quad, step): the four corners' attributes blended
@@ -401,6 +416,7 @@ __global__ __launch_bounds__(256) void k_setup(const float4* __restrict__ vtx,
                                                const float4* __restrict__ bounds,
                                                const float4* __restrict__ table, Geom G,
                                                Setup* __restrict__ setup, Attr* __restrict__ attr,
+                                               Edges* __restrict__ edges,
                                                uint32_t* __restrict__ counts) {
 #pragma clang fp contract(off)
     const long e = (long)blockIdx.x * 256 + threadIdx.x;
@@ -459,6 +475,11 @@ __global__ __launch_bounds__(256) void k_setup(const float4* __restrict__ vtx,
             flags |= is_top_left(vx[2], vy[2], vx[3], vy[3]) ? 4 : 0;
             flags |= is_top_left(vx[3], vy[3], vx[0], vy[0]) ? 8 : 0;
             flags |= is_top_left(vx[2], vy[2], vx[0], vy[0]) ? 16 : 0;
+            bool small = true;
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                small = small && abs(vx[j]) <= SMALL_COORD && abs(vy[j]) <= SMALL_COORD;
+            flags |= small ? FLAG_SMALL : 0;
             rec.flags = flags;
             // covered samples lie inside the integer vertices' bounding box
             long sx0 = min(min(vx[0], vx[1]), min(vx[2], vx[3]));
@@ -495,6 +516,22 @@ __global__ __launch_bounds__(256) void k_setup(const float4* __restrict__ vtx,
                     a = max(a, (long)tx0);
                     bb = min(bb, (long)tx1);
                     if (a <= bb) count += (uint32_t)(bb - a + 1);
+                }
+                if (count) {
+                    // edge rows 01, 12, 23, 30, 20 for the rasterizer
+                    Edges er;
+#pragma unroll
+                    for (int k = 0; k < 5; k++) {
+                        const int i = k < 4 ? k : 2, j = k < 3 ? k + 1 : 0;
+                        const uint32_t ea = (uint32_t)vx[i] - (uint32_t)vx[j];
+                        const uint32_t enb = (uint32_t)vy[j] - (uint32_t)vy[i];
+                        const uint32_t bias = ((flags >> k) & 1) ? 0u : 0xffffffffu;
+                        er.a[k] = (int)ea;
+                        er.nb[k] = (int)enb;
+                        er.c[k] = (int)(bias - ea * (uint32_t)vy[i] - enb * (uint32_t)vx[i]);
+                    }
+                    er.flags = flags;
+                    edges[e] = er;
                 }
             }
         }
@@ -597,6 +634,28 @@ __device__ __forceinline__ bool intersect_quad(int px, int py, const int* x, con
     return front || back;
 }
 
+// intersect_quad for a quad step with FLAG_SMALL, from its precomputed edge
+// rows: edge k at sample (px, py) is a_k py + nb_k px + c_k (mod 2^32), the
+// top-left bias folded into c_k.  The sample lies inside the image
+// (coordinates below 2^22: width and height are at most 2^19 pixels, AA at
+// most 8) and FLAG_SMALL bounds a_k and nb_k by 2^22, so both are signed
+// 24-bit operands and a 24-bit multiply returns the low 32 bits of the exact
+// product, the same wrapped value the 32-bit edge function computes.
+__device__ __forceinline__ int edge_row(const Edges& er, int k, int px, int py) {
+    return (int)((uint32_t)__mul24(er.a[k], py) + (uint32_t)__mul24(er.nb[k], px) + (uint32_t)er.c[k]);
+}
+
+__device__ __forceinline__ bool cover_small(int px, int py, const Edges& er) {
+    const bool w01 = edge_row(er, 0, px, py) >= 0;
+    const bool w12 = edge_row(er, 1, px, py) >= 0;
+    const bool w23 = edge_row(er, 2, px, py) >= 0;
+    const bool w30 = edge_row(er, 3, px, py) >= 0;
+    const bool w20 = edge_row(er, 4, px, py) >= 0;
+    const bool front = (w20 && w01 && w12 && (w23 || w30)) || (!w20 && w23 && w30 && (w01 || w12));
+    const bool back = (!w20 && !w01 && !w12 && (!w23 || !w30)) || (w20 && !w23 && !w30 && (!w01 || !w12));
+    return front || back;
+}
+
 // rasterizing.cl:95-148 (mean value coordinates of a quad)
 __device__ __forceinline__ void barycentric_quad(int px, int py, const int* x, const int* y,
                                                  float* w) {
@@ -636,53 +695,76 @@ __device__ __forceinline__ void barycentric_quad(int px, int py, const int* x, c
     for (int i = 0; i < 4; i++) w[i] = ww[i] / sum;
 }
 
-// Division and square root of the shading math at the precision OpenCL
-// guarantees for them (x/y within 2.5 ulp, sqrt within 3 ulp): a hardware
-// reciprocal or square root rounded once more.  The integer coverage test
-// does not use them.
-__device__ __forceinline__ float cl_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
-__device__ __forceinline__ float cl_sqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
-
 // rasterizing.cl:95-148 (mean value coordinates of a quad), shading precision:
-// approximate reciprocal and square root, fused multiply-adds allowed (the
+// hardware reciprocal square root, reciprocal and fused multiply-adds (the
 // weights only scale a fragment's colour; coverage is decided elsewhere in
-// integers).
+// integers).  1 / r_i comes from one rsq and r_i = |s|^2 / r_i, so the four
+// square roots and four divisions by r_i become four rsq.  A pixel on a
+// vertex (r_i == 0, as the integer squared distance shows) takes that
+// vertex's weight alone, as the source's early return does; it is handled
+// off the common path so the common path has no branches.
 __device__ __forceinline__ void barycentric_fast(int px, int py, const int* x, const int* y,
                                                  float* w) {
 #pragma clang fp contract(fast)
-    float sx[4], sy[4], a[4], d[4], r[4];
+    float sx[4], sy[4], q[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         sx[i] = (float)wsub(x[i], px);
         sy[i] = (float)wsub(y[i], py);
+        q[i] = sx[i] * sx[i] + sy[i] * sy[i];
+    }
+    if (q[0] == 0.0f || q[1] == 0.0f || q[2] == 0.0f || q[3] == 0.0f) {
+        const int z = q[0] == 0.0f ? 0 : (q[1] == 0.0f ? 1 : (q[2] == 0.0f ? 2 : 3));
+#pragma unroll
+        for (int k = 0; k < 4; k++) w[k] = (k == z) ? 1.0f : 0.0f;
+        return;
+    }
+    float ir[4], r[4], t[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        ir[i] = __builtin_amdgcn_rsqf(q[i]);
+        r[i] = q[i] * ir[i];
     }
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         const int j = (i + 1) & 3;
-        a[i] = sx[i] * sy[j] - sx[j] * sy[i];
-        d[i] = sx[i] * sx[j] + sy[i] * sy[j];
-        r[i] = cl_sqrt(sx[i] * sx[i] + sy[i] * sy[i]);
+        float a = sx[i] * sy[j] - sx[j] * sy[i];
+        const float d = sx[i] * sx[j] + sy[i] * sy[j];
+        a = a == 0.0f ? 1e-3f : a;
+        t[i] = (r[i] * r[j] - d) * __builtin_amdgcn_rcpf(a);
     }
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        if (r[i] == 0) {
-#pragma unroll
-            for (int k = 0; k < 4; k++) w[k] = (k == i) ? 1.0f : 0.0f;
-            return;
-        }
-    }
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-        if (a[i] == 0) a[i] = 1e-3f;
-    float t[4];
-#pragma unroll
-    for (int i = 0; i < 4; i++) t[i] = cl_div(r[i] * r[(i + 1) & 3] - d[i], a[i]);
     float ww[4];
 #pragma unroll
-    for (int i = 0; i < 4; i++) ww[i] = cl_div(t[(i + 3) & 3] + t[i], r[i]);
+    for (int i = 0; i < 4; i++) ww[i] = (t[(i + 3) & 3] + t[i]) * ir[i];
     const float inv = __builtin_amdgcn_rcpf(ww[0] + ww[1] + ww[2] + ww[3]);
 #pragma unroll
     for (int i = 0; i < 4; i++) w[i] = ww[i] * inv;
+}
+
+// Bilinear lookup of the one-channel ghost texture with border 0 (the OpenCL
+// 1.2 formula of sample_bilinear), for shading: no branches.  The texel
+// coordinate is clamped to [-2, size + 1] before the conversion, which keeps
+// every texel outside the texture outside it (so it still reads 0) and maps
+// NaN to a border texel (the weights are then NaN, as in the source).  Texels
+// outside the texture load texel 0 and are replaced by 0.
+__device__ __forceinline__ float ghost_texel(const float* __restrict__ tex, int w, int h, int i,
+                                             int j) {
+    const bool in = (uint32_t)i < (uint32_t)w && (uint32_t)j < (uint32_t)h;
+    const float v = tex[in ? j * w + i : 0];
+    return in ? v : 0.0f;
+}
+
+__device__ __forceinline__ float ghost_sample(const float* __restrict__ tex, int w, int h, float s,
+                                              float t) {
+#pragma clang fp contract(fast)
+    const float u = s * (float)w - 0.5f, v = t * (float)h - 0.5f;
+    const float fu = floorf(u), fv = floorf(v);
+    const int i0 = (int)fminf(fmaxf(fu, -2.0f), (float)w + 1.0f);
+    const int j0 = (int)fminf(fmaxf(fv, -2.0f), (float)h + 1.0f);
+    const float a = u - fu, b = v - fv;
+    const float t00 = ghost_texel(tex, w, h, i0, j0), t10 = ghost_texel(tex, w, h, i0 + 1, j0);
+    const float t01 = ghost_texel(tex, w, h, i0, j0 + 1), t11 = ghost_texel(tex, w, h, i0 + 1, j0 + 1);
+    return (1.0f - a) * (1.0f - b) * t00 + a * (1.0f - b) * t10 + (1.0f - a) * b * t01 + a * b * t11;
 }
 
 // Workgroups are dealt to the XCDs round-robin (workgroup i runs on XCD
@@ -722,10 +804,11 @@ __device__ __forceinline__ float shade_fragment(int pcx, int pcy, const Setup& r
     float vv = w[0] * at.v[0] + w[1] * at.v[1] + w[2] * at.v[2] + w[3] * at.v[3];
     uu = (uu * inv_scale + 1.0f) * 0.5f;
     vv = (vv * inv_scale + 1.0f) * 0.5f;
-    const float gi = sample_bilinear(ghost, G.gw, G.gh, 1, uu, vv, false);
+    const float gi = ghost_sample(ghost, G.gw, G.gh, uu, vv);
     const float rel = w[0] * at.rrel[0] + w[1] * at.rrel[1] + w[2] * at.rrel[2] + w[3] * at.rrel[3];
     // smoothstep(1, 0.95, rrel)
-    const float tt = clampn(cl_div(rel - 1.0f, 0.95f - 1.0f), 0.0f, 1.0f);
+    constexpr float inv_edge = 1.0f / (0.95f - 1.0f);
+    const float tt = clampn((rel - 1.0f) * inv_edge, 0.0f, 1.0f);
     const float ri = tt * tt * (3.0f - 2.0f * tt);
     float ai = w[0] * at.inten[0] + w[1] * at.inten[1] + w[2] * at.inten[2] + w[3] * at.inten[3];
     float ci = w[0] * at.refl[0] + w[1] * at.refl[1] + w[2] * at.refl[2] + w[3] * at.refl[3];
@@ -753,6 +836,7 @@ __device__ __forceinline__ float lane_value(float v, uint32_t src_lane) {
 __global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ bounds,
                                                const float* __restrict__ ghost,
                                                const Setup* __restrict__ setup,
+                                               const Edges* __restrict__ edges,
                                                const Attr* __restrict__ attr,
                                                const uint32_t* __restrict__ vals,
                                                const uint32_t* __restrict__ tstart,
@@ -817,11 +901,11 @@ __global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ bounds
     const uint32_t start = tstart[tile], end = tend[tile];
     for (uint32_t k = start; k < end; k++) {
         const uint32_t e = vals[k];
-        const Setup rec = setup[e];
+        const Edges er = edges[e];
         int hits = 0;
         bool ok = inside;
         if (ok && !G.bin_exact) {
-            const float4 b = bounds[rec.prim];
+            const float4 b = bounds[setup[e].prim];
             const int sbx0 = wadd(rf_to_int(b.x * G.st), G.width / 2);
             const int sby0 = wadd(rf_to_int(b.y * G.st), G.height / 2);
             const int sbx1 = wadd(rf_to_int(b.z * G.st), G.width / 2);
@@ -830,9 +914,19 @@ __global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ bounds
             ok = sbx0 <= bx0 + G.bin && sbx1 >= bx0 && sby0 <= by0 + G.bin && sby1 >= by0;
         }
         if (ok) {
-            for (int s = 0; s < G.AA; s++) {
-                const int sx = px + s, sy = py + sub_offset(G.AA - 1 + s);
-                if (intersect_quad(sx, sy, rec.vx, rec.vy, rec.flags)) hits++;
+            if (er.flags & FLAG_SMALL) {
+                if (G.AA == 1) {
+                    hits = cover_small(px, py, er) ? 1 : 0;
+                } else {
+                    for (int s = 0; s < G.AA; s++)
+                        hits += cover_small(px + s, py + sub_offset(G.AA - 1 + s), er) ? 1 : 0;
+                }
+            } else {
+                const Setup rec = setup[e];
+                for (int s = 0; s < G.AA; s++) {
+                    const int sx = px + s, sy = py + sub_offset(G.AA - 1 + s);
+                    if (intersect_quad(sx, sy, rec.vx, rec.vy, rec.flags)) hits++;
+                }
             }
         }
         const uint64_t m = __ballot(hits > 0);
@@ -935,8 +1029,8 @@ static int make_geom(const rf_raster_params* p, Geom* G) {
 static inline unsigned blocks_for(long n, int threads) { return (unsigned)((n + threads - 1) / threads); }
 
 struct Layout {
-    size_t header, table, setup, attr, counts, offsets, scan_ws, keys, vals, radix_ws, tstart, tend,
-        total;
+    size_t header, table, setup, attr, edges, counts, offsets, scan_ws, keys, vals, radix_ws, tstart,
+        tend, total;
 };
 
 static Layout layout(const Geom& G, uint64_t cap) {
@@ -951,6 +1045,7 @@ static Layout layout(const Geom& G, uint64_t cap) {
     L.table = take((size_t)G.S * sizeof(float4));
     L.setup = take((size_t)G.NE * sizeof(Setup));
     L.attr = take((size_t)G.NE * sizeof(Attr));
+    L.edges = take((size_t)G.NE * sizeof(Edges));
     L.counts = take((size_t)G.NE * sizeof(uint32_t));
     L.offsets = take((size_t)G.NE * sizeof(uint64_t));
     L.scan_ws = take(scan_workspace_bytes((uint64_t)G.NE));
@@ -969,6 +1064,7 @@ struct RasterPtrs {
     float4* table;
     Setup* setup;
     Attr* attr;
+    Edges* edges;
     uint32_t *counts, *keys, *vals, *tstart, *tend;
     uint64_t* offsets;
     void *scan_ws, *radix_ws;
@@ -985,6 +1081,7 @@ static int raster_ptrs(const Geom& G, uint64_t cap, void* ws, size_t ws_bytes, R
     r->table = (float4*)(w + L.table);
     r->setup = (Setup*)(w + L.setup);
     r->attr = (Attr*)(w + L.attr);
+    r->edges = (Edges*)(w + L.edges);
     r->counts = (uint32_t*)(w + L.counts);
     r->offsets = (uint64_t*)(w + L.offsets);
     r->scan_ws = w + L.scan_ws;
@@ -1003,7 +1100,7 @@ static int bin_impl(const float4* vtx, const float4* bounds, const float* spectr
     hipLaunchKernelGGL(k_steps, dim3(blocks_for(G.maxw, 64)), dim3(64), 0, s, spectrum, G, r.table);
     if (G.NE > 0) {
         hipLaunchKernelGGL(k_setup, dim3(blocks_for(G.NE, 256)), dim3(256), 0, s, vtx, bounds,
-                           r.table, G, r.setup, r.attr, r.counts);
+                           r.table, G, r.setup, r.attr, r.edges, r.counts);
     }
     int rc = scan_exclusive_u32(r.counts, r.offsets, nullptr, (uint64_t)G.NE, &r.h->required,
                                 r.scan_ws, s);
@@ -1034,7 +1131,7 @@ static int draw_impl(const float4* vtx, const float4* bounds, const float* ghost
     (void)vtx;
     (void)ntiles;
     hipLaunchKernelGGL(k_raster, dim3((unsigned)(raster_grid(G) * G.frames)), dim3(64), 0, s, bounds,
-                       ghost, r.setup,
+                       ghost, r.setup, r.edges,
                        r.attr, r.vals, r.tstart, r.tend, r.h, G, image);
     RF_LAUNCH_CHECK("rasterizer");
     return RF_OK;
