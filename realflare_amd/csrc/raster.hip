@@ -745,16 +745,19 @@ __device__ __forceinline__ void barycentric_fast(int px, int py, const int* x, c
 // 1.2 formula of sample_bilinear), for shading: no branches.  The texel
 // coordinate is clamped to [-2, size + 1] before the conversion, which keeps
 // every texel outside the texture outside it (so it still reads 0) and maps
-// NaN to a border texel (the weights are then NaN, as in the source).  Texels
-// outside the texture load texel 0 and are replaced by 0.
-__device__ __forceinline__ float ghost_texel(const float* __restrict__ tex, int w, int h, int i,
-                                             int j) {
+// NaN to a border texel (the weights are then NaN, as in the source).  The
+// texture is read through a buffer resource sized to it: a texel outside the
+// texture gets an offset past the end, which the hardware range check
+// answers with 0.
+static constexpr int RF_BUFFER_WORD3 = 0x00020000;  // raw 32-bit buffer, gfx9 descriptor
+
+__device__ __forceinline__ float ghost_texel(__amdgpu_buffer_rsrc_t tex, int w, int h, int i, int j) {
     const bool in = (uint32_t)i < (uint32_t)w && (uint32_t)j < (uint32_t)h;
-    const float v = tex[in ? j * w + i : 0];
-    return in ? v : 0.0f;
+    const uint32_t off = in ? (uint32_t)(j * w + i) * 4u : 0x80000000u;
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(tex, off, 0, 0));
 }
 
-__device__ __forceinline__ float ghost_sample(const float* __restrict__ tex, int w, int h, float s,
+__device__ __forceinline__ float ghost_sample(__amdgpu_buffer_rsrc_t tex, int w, int h, float s,
                                               float t) {
 #pragma clang fp contract(fast)
     const float u = s * (float)w - 0.5f, v = t * (float)h - 0.5f;
@@ -795,7 +798,7 @@ __device__ __forceinline__ bool raster_tile(unsigned bid, const Geom& G, int* tx
 // fragment_shader (rasterizing.cl:471-506) for one covered pixel of one quad
 // step, scaled by its covered AA samples; fused multiply-adds allowed.
 __device__ __forceinline__ float shade_fragment(int pcx, int pcy, const Setup& rec, const Attr& at,
-                                                int hits, const float* __restrict__ ghost,
+                                                int hits, __amdgpu_buffer_rsrc_t ghost,
                                                 const Geom& G, float inv_scale) {
 #pragma clang fp contract(fast)
     float w[4];
@@ -821,9 +824,6 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
 }
 
-__device__ __forceinline__ float lane_value(float v, uint32_t src_lane) {
-    return __int_as_float(__builtin_amdgcn_ds_bpermute((int)(src_lane << 2), __float_as_int(v)));
-}
 
 // rasterizer (rasterizing.cl:524-663): one wave per 8x8 tile, lane = pixel.
 // The tile's (quad step) pairs are walked in binning order.  A pair's coverage
@@ -845,6 +845,8 @@ __global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ bounds
                                                float4* __restrict__ image) {
 #pragma clang fp contract(off)
     __shared__ uint32_t s_q[64];  // pixel lane | hits << 6 | slot << 10
+    __shared__ uint4 s_slot[64];  // per queued pair: coverage mask lo, hi, first queue index, entry
+    __shared__ float4 s_c[64];    // shaded colour of each queue entry
     if (h->overflow) return;
     const long per_frame = raster_grid(G);
     const int frame = (int)(blockIdx.x / per_frame);
@@ -858,10 +860,10 @@ __global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ bounds
     const int bxp = x / G.bin, byp = y / G.bin;
     const float inv_scale = 1.0f / G.ghost_scale;
     float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f;
-    // queue state (wave-uniform) and per-slot records, slot s held by lane s
+    // queue state (wave-uniform); slot s (the s-th queued pair) in s_slot[s]
     uint32_t qlen = 0, nslot = 0;
-    uint64_t slot_mask = 0;
-    uint32_t slot_lo = 0, slot_e = 0;
+    const __amdgpu_buffer_rsrc_t ghost_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)ghost, (short)0, G.gw * G.gh * (int)sizeof(float), RF_BUFFER_WORD3);
 
     auto flush = [&]() {
         __syncthreads();  // queue entries written by other lanes are visible
@@ -869,28 +871,26 @@ __global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ bounds
         if ((uint32_t)lane < qlen) {
             const uint32_t ent = s_q[lane];
             const int p = ent & 63, hits = (ent >> 6) & 15;
-            const uint32_t e = (uint32_t)__builtin_amdgcn_ds_bpermute((int)((ent >> 10) << 2),
-                                                                     (int)slot_e);
+            const uint32_t e = s_slot[ent >> 10].w;
             const Setup rec = setup[e];
             const Attr at = attr[e];
             const int fx = tx * TILE + (p & (TILE - 1)), fy = ty * TILE + p / TILE;
             const float frag = shade_fragment(fx * G.AA + G.AA / 2, fy * G.AA + G.AA / 2, rec, at,
-                                              hits, ghost, G, inv_scale);
+                                              hits, ghost_rsrc, G, inv_scale);
             c0 = at.x * frag;
             c1 = at.y * frag;
             c2 = at.z * frag;
         }
+        s_c[lane] = make_float4(c0, c1, c2, 0.0f);
+        __syncthreads();  // colours visible to their pixel lanes
         for (uint32_t sl = 0; sl < nslot; sl++) {
-            const uint32_t mlo = __builtin_amdgcn_readlane((uint32_t)slot_mask, sl);
-            const uint32_t mhi = __builtin_amdgcn_readlane((uint32_t)(slot_mask >> 32), sl);
-            const uint32_t lo = __builtin_amdgcn_readlane(slot_lo, sl);
-            const uint64_t m = ((uint64_t)mhi << 32) | mlo;
-            const uint32_t src = lo + lanes_below(m);
-            const float v0 = lane_value(c0, src), v1 = lane_value(c1, src), v2 = lane_value(c2, src);
+            const uint4 si = s_slot[sl];
+            const uint64_t m = ((uint64_t)si.y << 32) | si.x;
             if ((m >> lane) & 1) {
-                r0 += v0;
-                r1 += v1;
-                r2 += v2;
+                const float4 v = s_c[si.z + lanes_below(m)];
+                r0 += v.x;
+                r1 += v.y;
+                r2 += v.z;
             }
         }
         qlen = 0;
@@ -934,11 +934,7 @@ __global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ bounds
         const uint32_t n = (uint32_t)__popcll(m);
         if (qlen + n > 64) flush();
         if (hits > 0) s_q[qlen + lanes_below(m)] = (uint32_t)lane | (uint32_t)hits << 6 | nslot << 10;
-        if ((uint32_t)lane == nslot) {
-            slot_mask = m;
-            slot_lo = qlen;
-            slot_e = e;
-        }
+        if (lane == 0) s_slot[nslot] = make_uint4((uint32_t)m, (uint32_t)(m >> 32), qlen, e);
         qlen += n;
         nslot++;
     }
@@ -976,6 +972,9 @@ static int make_geom(const rf_raster_params* p, Geom* G) {
     if (p->bin_size < 1) return set_error(RF_ERR_INVALID, "raster: bin_size must be >= 1");
     if (p->ghost_width < 1 || p->ghost_height < 1)
         return set_error(RF_ERR_INVALID, "raster: ghost texture size must be positive");
+    if ((long)p->ghost_width * p->ghost_height > (1L << 28))
+        return set_error(RF_ERR_INVALID, "raster: ghost texture of %dx%d texels exceeds 2^28",
+                         p->ghost_width, p->ghost_height);
     G->P = p->path_count;
     G->W = p->wavelength_count;
     G->g = p->grid_count;
