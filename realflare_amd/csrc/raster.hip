@@ -757,17 +757,19 @@ __device__ __forceinline__ float ghost_texel(__amdgpu_buffer_rsrc_t tex, int w, 
     return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(tex, off, 0, 0));
 }
 
-__device__ __forceinline__ float ghost_sample(__amdgpu_buffer_rsrc_t tex, int w, int h, float s,
-                                              float t) {
+// (u, v) in texel units: s * w - 0.5, t * h - 0.5.  Blended as two lerps
+// along u and one along v (the source's four-product form, reassociated).
+__device__ __forceinline__ float ghost_sample(__amdgpu_buffer_rsrc_t tex, int w, int h, float u,
+                                              float v) {
 #pragma clang fp contract(fast)
-    const float u = s * (float)w - 0.5f, v = t * (float)h - 0.5f;
     const float fu = floorf(u), fv = floorf(v);
     const int i0 = (int)fminf(fmaxf(fu, -2.0f), (float)w + 1.0f);
     const int j0 = (int)fminf(fmaxf(fv, -2.0f), (float)h + 1.0f);
     const float a = u - fu, b = v - fv;
     const float t00 = ghost_texel(tex, w, h, i0, j0), t10 = ghost_texel(tex, w, h, i0 + 1, j0);
     const float t01 = ghost_texel(tex, w, h, i0, j0 + 1), t11 = ghost_texel(tex, w, h, i0 + 1, j0 + 1);
-    return (1.0f - a) * (1.0f - b) * t00 + a * (1.0f - b) * t10 + (1.0f - a) * b * t01 + a * b * t11;
+    const float top = t00 + a * (t10 - t00), bottom = t01 + a * (t11 - t01);
+    return top + b * (bottom - top);
 }
 
 // Workgroups are dealt to the XCDs round-robin (workgroup i runs on XCD
@@ -797,27 +799,37 @@ __device__ __forceinline__ bool raster_tile(unsigned bid, const Geom& G, int* tx
 
 // fragment_shader (rasterizing.cl:471-506) for one covered pixel of one quad
 // step, scaled by its covered AA samples; fused multiply-adds allowed.
+// Ghost texture lookup constants: texel coordinate = uv * k + c, folding the
+// (uv / ghost_scale + 1) / 2 of rasterizing.cl and the texel-centre offset.
+struct GhostMap {
+    float ku, cu, kv, cv;
+};
+
+__device__ __forceinline__ GhostMap ghost_map(const Geom& G) {
+    const float half_inv = 0.5f / G.ghost_scale;
+    return {half_inv * (float)G.gw, 0.5f * (float)G.gw - 0.5f, half_inv * (float)G.gh,
+            0.5f * (float)G.gh - 0.5f};
+}
+
 __device__ __forceinline__ float shade_fragment(int pcx, int pcy, const Setup& rec, const Attr& at,
                                                 int hits, __amdgpu_buffer_rsrc_t ghost,
-                                                const Geom& G, float inv_scale) {
+                                                const Geom& G, const GhostMap& gm) {
 #pragma clang fp contract(fast)
     float w[4];
     barycentric_fast(pcx, pcy, rec.vx, rec.vy, w);
-    float uu = w[0] * at.u[0] + w[1] * at.u[1] + w[2] * at.u[2] + w[3] * at.u[3];
-    float vv = w[0] * at.v[0] + w[1] * at.v[1] + w[2] * at.v[2] + w[3] * at.v[3];
-    uu = (uu * inv_scale + 1.0f) * 0.5f;
-    vv = (vv * inv_scale + 1.0f) * 0.5f;
-    const float gi = ghost_sample(ghost, G.gw, G.gh, uu, vv);
+    const float uu = w[0] * at.u[0] + w[1] * at.u[1] + w[2] * at.u[2] + w[3] * at.u[3];
+    const float vv = w[0] * at.v[0] + w[1] * at.v[1] + w[2] * at.v[2] + w[3] * at.v[3];
+    const float gi = ghost_sample(ghost, G.gw, G.gh, uu * gm.ku + gm.cu, vv * gm.kv + gm.cv);
     const float rel = w[0] * at.rrel[0] + w[1] * at.rrel[1] + w[2] * at.rrel[2] + w[3] * at.rrel[3];
     // smoothstep(1, 0.95, rrel)
     constexpr float inv_edge = 1.0f / (0.95f - 1.0f);
-    const float tt = clampn((rel - 1.0f) * inv_edge, 0.0f, 1.0f);
+    const float tt = clampn(rel * inv_edge - inv_edge, 0.0f, 1.0f);
     const float ri = tt * tt * (3.0f - 2.0f * tt);
     float ai = w[0] * at.inten[0] + w[1] * at.inten[1] + w[2] * at.inten[2] + w[3] * at.inten[3];
     float ci = w[0] * at.refl[0] + w[1] * at.refl[1] + w[2] * at.refl[2] + w[3] * at.refl[3];
     ai = fmaxf(ai, 0.0f);
     ci = fmaxf(ci, 0.0f);
-    return 0.0f + gi * ri * ai * ci * (float)hits;
+    return gi * ri * ai * ci * (float)hits;
 }
 
 __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
@@ -858,7 +870,7 @@ __global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ bounds
     const bool inside = x < G.width && y < G.height;
     const int px = x * G.AA, py = y * G.AA;
     const int bxp = x / G.bin, byp = y / G.bin;
-    const float inv_scale = 1.0f / G.ghost_scale;
+    const GhostMap gmap = ghost_map(G);
     float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f;
     // queue state (wave-uniform); slot s (the s-th queued pair) in s_slot[s]
     uint32_t qlen = 0, nslot = 0;
@@ -876,7 +888,7 @@ __global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ bounds
             const Attr at = attr[e];
             const int fx = tx * TILE + (p & (TILE - 1)), fy = ty * TILE + p / TILE;
             const float frag = shade_fragment(fx * G.AA + G.AA / 2, fy * G.AA + G.AA / 2, rec, at,
-                                              hits, ghost_rsrc, G, inv_scale);
+                                              hits, ghost_rsrc, G, gmap);
             c0 = at.x * frag;
             c1 = at.y * frag;
             c2 = at.z * frag;
