@@ -355,6 +355,8 @@ def fit_timing(timed_steps: int = 3, fit_steps: int = 100) -> dict:
     fs.set_targets(target)
     _, hist = fit(fs, [i.params for i in small], fit_steps)
     losses_fit = [h['loss'] for h in hist]
+    first, best = hist[0]['instance_loss'], hist[-1]['instance_best']
+    ratio = [b / f if f > 0 else 0.0 for f, b in zip(first, best)]
     return {
         'lens_instances': len(inst), 'surfaces': [len(m.lens_elements) - 1 for m in models],
         'ghosts': ghosts, 'light_positions': len(positions), 'wavelengths': len(wavelengths),
@@ -365,6 +367,9 @@ def fit_timing(timed_steps: int = 3, fit_steps: int = 100) -> dict:
                 'loss_best': hist[-1]['best_loss'],
                 'loss_at': {str(k): hist[k]['best_loss'] for k in (0, 10, 25, 50, 99)
                             if k < len(hist)},
+                'instance_loss_first': first, 'instance_loss_best': best,
+                'instances_reduced_10x': sum(r <= 0.1 for r in ratio),
+                'instance_best_over_first_median': float(sorted(ratio)[len(ratio) // 2]),
                 'ms_per_step_mean': sum(h['gpu_ms'] for h in hist) / len(hist)},
     }
 

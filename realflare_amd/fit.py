@@ -186,31 +186,41 @@ class FitBatch:
 
 
 class Adam:
-    """Adam on parameters measured relative to their initial magnitude."""
+    """Adam on parameters measured relative to their initial magnitude, with
+    one step size per lens instance."""
 
     def __init__(self, params: list[LensParams], lr: float = 1e-4, betas=(0.9, 0.999),
                  eps: float = 1e-12) -> None:
-        self.lr, self.b1, self.b2, self.eps = lr, betas[0], betas[1], eps
+        self.lr = [lr] * len(params)
+        self.b1, self.b2, self.eps = betas[0], betas[1], eps
         self.scale = [self._scale(p) for p in params]
         self.m = [np.zeros_like(s) for s in self.scale]
         self.v = [np.zeros_like(s) for s in self.scale]
-        self.t = 0
+        self.t = [0] * len(params)
 
     @staticmethod
     def _scale(p: LensParams) -> np.ndarray:
         f = p.flat().detach().double().numpy()
         return np.where(f != 0, np.abs(f), 1.0)
 
+    def restart(self, k: int, lr: float) -> None:
+        """Forget instance k's moments and continue with step size lr."""
+        self.lr[k] = lr
+        self.m[k][:] = 0
+        self.v[k][:] = 0
+        self.t[k] = 0
+
     def update(self, params: list[LensParams], grads) -> list[LensParams]:
-        self.t += 1
         out = []
         for k, (p, g) in enumerate(zip(params, grads)):
-            gf = torch.cat([t.reshape(-1) for t in g]).double().numpy() * self.scale[k]
+            self.t[k] += 1
+            t = self.t[k]
+            gf = torch.cat([x.reshape(-1) for x in g]).double().numpy() * self.scale[k]
             self.m[k] = self.b1 * self.m[k] + (1 - self.b1) * gf
             self.v[k] = self.b2 * self.v[k] + (1 - self.b2) * gf * gf
-            mh = self.m[k] / (1 - self.b1 ** self.t)
-            vh = self.v[k] / (1 - self.b2 ** self.t)
-            step = self.lr * mh / (np.sqrt(vh) + self.eps) * self.scale[k]
+            mh = self.m[k] / (1 - self.b1 ** t)
+            vh = self.v[k] / (1 - self.b2 ** t)
+            step = self.lr[k] * mh / (np.sqrt(vh) + self.eps) * self.scale[k]
             flat = p.flat().detach().double().numpy() - step
             out.append(LensParams.unflat(torch.tensor(flat), p))
         return out
@@ -239,15 +249,21 @@ def make_instances(models: list[LensModel], positions, grid_range=(64, 128), sen
 
 
 def fit(batch: FitBatch, start: list[LensParams], steps: int, lr: float = 1e-4,
-        on_step=None) -> tuple[list[LensParams], list[dict]]:
-    """`steps` Adam steps from `start`.  The loss is not convex (ghost rays
-    appear and vanish at aperture edges), so steps that raise it are kept
-    (they can cross such a barrier) and the best parameters seen are
-    returned.  History: loss of every evaluated point, best loss so far, GPU
-    time.  `on_step(step, params)` runs after each gradient evaluation."""
+        on_step=None, patience: float = 2.0) -> tuple[list[LensParams], list[dict]]:
+    """`steps` Adam steps from `start`.  The instances are independent, so
+    each keeps its own best point and step size.  The loss is not convex
+    (ghost rays appear and vanish at aperture edges), so a step that raises an
+    instance's loss is kept unless the loss exceeds `patience` times that
+    instance's best; then the instance returns to its best point with half the
+    step size.  `on_step(step, params)` runs after each gradient
+    evaluation.  Returns the best parameters of every instance.  History:
+    summed and per-instance losses of every evaluated point and the best
+    losses so far, GPU time."""
     opt = Adam(start, lr)
-    params = start
-    best = None
+    params = list(start)
+    n = len(start)
+    best_p = list(start)
+    best_l = [float('inf')] * n
     hist = []
     for s in range(steps):
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -255,13 +271,22 @@ def fit(batch: FitBatch, start: list[LensParams], steps: int, lr: float = 1e-4,
         losses, grads = batch.step(params)
         b.record()
         torch.cuda.synchronize()
-        loss = float(losses.sum())
-        rec = {'step': s, 'loss': loss, 'gpu_ms': a.elapsed_time(b)}
+        per = [float(x) for x in losses.reshape(-1)]
+        rec = {'step': s, 'loss': sum(per), 'gpu_ms': a.elapsed_time(b)}
         if on_step is not None:
             rec.update(on_step(s, params) or {})
-        if best is None or loss < best[1]:
-            best = (params, loss)
-        rec['best_loss'] = best[1]
+        restart = []
+        for k in range(n):
+            if per[k] < best_l[k]:
+                best_l[k], best_p[k] = per[k], params[k]
+            elif per[k] > patience * best_l[k]:
+                restart.append(k)
+        rec['best_loss'] = sum(best_l)
+        rec['instance_loss'] = per
+        rec['instance_best'] = list(best_l)
         hist.append(rec)
         params = opt.update(params, grads)
-    return best[0], hist
+        for k in restart:
+            params[k] = best_p[k]
+            opt.restart(k, opt.lr[k] * 0.5)
+    return best_p, hist
