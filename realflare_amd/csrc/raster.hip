@@ -116,7 +116,7 @@ struct Header {
     uint64_t required;  // (tile, quad-step) pairs the frame needs
     uint64_t n_sort;    // pairs binned (0 on overflow)
     uint32_t overflow;  // required > capacity: nothing was rastered
-    uint32_t pad_;
+    uint32_t heavy;     // quad steps left to k_expand_heavy (more than HEAVY_PAIRS pairs)
 };
 
 // Setup.flags bit 5: every vertex coordinate lies within +-SMALL_COORD
@@ -544,7 +544,14 @@ __global__ void k_finalize(Header* __restrict__ h, uint64_t capacity) {
     const uint64_t need = h->required;
     h->overflow = need > capacity ? 1u : 0u;
     h->n_sort = need > capacity ? 0 : need;
+    h->heavy = 0;
 }
+
+// Pairs of one quad step above which its expansion is left to a whole
+// workgroup (k_expand_heavy): a few quad steps cover thousands of tiles, and
+// one lane walking them would hold its wave for thousands of iterations.
+static constexpr uint32_t HEAVY_PAIRS = 1024;
+static constexpr int HEAVY_BLOCKS = 1024;  // persistent grid of k_expand_heavy
 
 // One thread per (quad, step): walks the tile rows of its rectangle with the
 // hull span k_setup counted with.  A block's 256 entries own one contiguous
@@ -578,9 +585,10 @@ __device__ __forceinline__ void expand_entry(const Setup& rec, const Geom& G, ui
 __global__ __launch_bounds__(256) void k_expand(const Setup* __restrict__ setup,
                                                 const uint32_t* __restrict__ counts,
                                                 const uint64_t* __restrict__ offsets,
-                                                const Header* __restrict__ h, Geom G,
+                                                Header* __restrict__ h, Geom G,
                                                 uint32_t* __restrict__ keys,
-                                                uint32_t* __restrict__ vals) {
+                                                uint32_t* __restrict__ vals,
+                                                uint32_t* __restrict__ heavy) {
     __shared__ uint32_t s_keys[EXPAND_STAGE], s_vals[EXPAND_STAGE];
     if (h->overflow) return;
     const long first = (long)blockIdx.x * 256;
@@ -589,7 +597,11 @@ __global__ __launch_bounds__(256) void k_expand(const Setup* __restrict__ setup,
     const uint64_t begin = offsets[first];
     const uint64_t end = offsets[last] + counts[last];
     const bool staged = end - begin <= EXPAND_STAGE;  // block-uniform
-    if (e <= last && counts[e]) {
+    const uint32_t cnt = e <= last ? counts[e] : 0u;
+    if (cnt > HEAVY_PAIRS) {
+        // its run is written by k_expand_heavy, after the staged copy below
+        heavy[atomicAdd(&h->heavy, 1u)] = (uint32_t)e;
+    } else if (cnt) {
         uint32_t* kp = staged ? s_keys : keys;  // generic pointers: LDS or global
         uint32_t* vp = staged ? s_vals : vals;
         expand_entry(setup[e], G, (uint32_t)e, offsets[e] - (staged ? begin : 0), kp, vp);
@@ -600,6 +612,43 @@ __global__ __launch_bounds__(256) void k_expand(const Setup* __restrict__ setup,
     for (uint32_t i = threadIdx.x; i < n; i += 256) {
         keys[begin + i] = s_keys[i];
         vals[begin + i] = s_vals[i];
+    }
+}
+
+// The quad steps k_expand deferred, one workgroup at a time: rows in order,
+// each row's span computed as k_setup computed it (every thread redundantly),
+// its tiles written by consecutive threads.  The output positions are fixed by
+// the scan, so the order the list was filled in does not matter.
+__global__ __launch_bounds__(256) void k_expand_heavy(const Setup* __restrict__ setup,
+                                                      const uint64_t* __restrict__ offsets,
+                                                      const Header* __restrict__ h, Geom G,
+                                                      const uint32_t* __restrict__ heavy,
+                                                      uint32_t* __restrict__ keys,
+                                                      uint32_t* __restrict__ vals) {
+    if (h->overflow) return;
+    const uint32_t n = h->heavy;
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+        const uint32_t e = heavy[i];
+        const Setup rec = setup[e];
+        uint64_t off = offsets[e];
+        const uint32_t tx0 = rec.r0 & 0xffff, ty0 = rec.r0 >> 16;
+        const uint32_t tx1 = rec.r1 & 0xffff, ty1 = rec.r1 >> 16;
+        HullRows hull;
+        hull_prepare(rec.vx, rec.vy, &hull);
+        const uint32_t frame_key =
+            G.batch ? (uint32_t)loc_quad(G, rec.prim).frame * (uint32_t)(G.ntx * G.nty) : 0u;
+        for (uint32_t ty = ty0; ty <= ty1; ty++) {
+            long a, b;
+            hull_row_span(hull, G.AA, (int)ty, &a, &b);
+            a = max(a, (long)tx0);
+            b = min(b, (long)tx1);
+            if (a > b) continue;
+            for (long tx = a + threadIdx.x; tx <= b; tx += 256) {
+                keys[off + (tx - a)] = frame_key + ty * (uint32_t)G.ntx + (uint32_t)tx;
+                vals[off + (tx - a)] = e;
+            }
+            off += (uint64_t)(b - a + 1);
+        }
     }
 }
 
@@ -1040,7 +1089,7 @@ static int make_geom(const rf_raster_params* p, Geom* G) {
 static inline unsigned blocks_for(long n, int threads) { return (unsigned)((n + threads - 1) / threads); }
 
 struct Layout {
-    size_t header, table, setup, attr, edges, counts, offsets, scan_ws, keys, vals, radix_ws, tstart,
+    size_t header, table, setup, attr, edges, heavy, counts, offsets, scan_ws, keys, vals, radix_ws, tstart,
         tend, total;
 };
 
@@ -1057,6 +1106,7 @@ static Layout layout(const Geom& G, uint64_t cap) {
     L.setup = take((size_t)G.NE * sizeof(Setup));
     L.attr = take((size_t)G.NE * sizeof(Attr));
     L.edges = take((size_t)G.NE * sizeof(Edges));
+    L.heavy = take((size_t)G.NE * sizeof(uint32_t));
     L.counts = take((size_t)G.NE * sizeof(uint32_t));
     L.offsets = take((size_t)G.NE * sizeof(uint64_t));
     L.scan_ws = take(scan_workspace_bytes((uint64_t)G.NE));
@@ -1076,6 +1126,7 @@ struct RasterPtrs {
     Setup* setup;
     Attr* attr;
     Edges* edges;
+    uint32_t* heavy;
     uint32_t *counts, *keys, *vals, *tstart, *tend;
     uint64_t* offsets;
     void *scan_ws, *radix_ws;
@@ -1093,6 +1144,7 @@ static int raster_ptrs(const Geom& G, uint64_t cap, void* ws, size_t ws_bytes, R
     r->setup = (Setup*)(w + L.setup);
     r->attr = (Attr*)(w + L.attr);
     r->edges = (Edges*)(w + L.edges);
+    r->heavy = (uint32_t*)(w + L.heavy);
     r->counts = (uint32_t*)(w + L.counts);
     r->offsets = (uint64_t*)(w + L.offsets);
     r->scan_ws = w + L.scan_ws;
@@ -1119,7 +1171,9 @@ static int bin_impl(const float4* vtx, const float4* bounds, const float* spectr
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1), 0, s, r.h, cap);
     if (G.NE > 0) {
         hipLaunchKernelGGL(k_expand, dim3(blocks_for(G.NE, 256)), dim3(256), 0, s, r.setup, r.counts,
-                           r.offsets, r.h, G, r.keys, r.vals);
+                           r.offsets, r.h, G, r.keys, r.vals, r.heavy);
+        hipLaunchKernelGGL(k_expand_heavy, dim3(HEAVY_BLOCKS), dim3(256), 0, s, r.setup, r.offsets,
+                           r.h, G, r.heavy, r.keys, r.vals);
     }
     int bits = 1;
     while ((1L << bits) < ntiles) bits++;

@@ -1,7 +1,9 @@
 """Parity of the HIP rasterizer (rf_prim_shader, rf_vertex_shader,
 rf_rasterize, rf_flare) with the oracle's restatement of rasterizing.cl.
 
-* Same rays in: the GPU trace's planes are fed to both rasterizers.  Bounds,
+* Same rays in: the GPU trace's planes are fed to both rasterizers (a 48^2
+  grid at 512^2, and a 4^2 grid at 1024^2 whose quads each cover thousands of
+  tiles).  Bounds,
   intensities and vertices must be bit-identical (same IEEE operations, no
   contraction); the image must match within 1e-6 relative L1 (the oracle
   and the kernel evaluate the same expressions; differences can only come
@@ -32,14 +34,15 @@ def _nan_equal(a, b):
     return np.array_equal(np.isnan(a), np.isnan(b)) and np.array_equal(a[~np.isnan(a)], b[~np.isnan(b)])
 
 
-def test_raster_stages_same_rays(oracle_lib):
+@pytest.mark.parametrize('g,res', [(48, 512), (4, 1024)])
+def test_raster_stages_same_rays(g, res, oracle_lib):
     from realflare_amd import native
 
-    inp = trace_inputs('small', grid_count=48)
-    print('case raster/small same-rays g=48 512^2', flush=True)
+    inp = trace_inputs('small', grid_count=g)
+    print(f'case raster/small same-rays g={g} {res}^2', flush=True)
     planes = gpu_trace_planes(inp)
     ghost = make_ghost(oracle_lib)
-    prm = params_for(inp, (512, 512), ghost.shape)
+    prm = params_for(inp, (res, res), ghost.shape)
     rays = planes_to_oracle_rays(planes.cpu().numpy(), oracle_lib)
     ob, oi, ov, oimg = oracle_image(oracle_lib, rays, inp, prm, ghost)
 
@@ -53,6 +56,27 @@ def test_raster_stages_same_rays(oracle_lib):
     native.call('rf_vertex_shader', native.ptr(planes), native.ptr(intens), ctypes.byref(prm),
                 native.ptr(vtx), native.stream())
     gb, gi, gv = bounds.cpu().numpy(), intens.cpu().numpy(), vtx.cpu().numpy()
+    if g == 4:
+        # some quad this coarse covers the centres of more than 1024 tiles of
+        # 8x8 pixels, the size above which the binner expands a quad step
+        # with a whole workgroup
+        pos = ov['pos'].reshape(P, g, g, W, 2)
+        c = [pos[:, :-1, :-1], pos[:, :-1, 1:], pos[:, 1:, 1:], pos[:, 1:, :-1]]
+        valid = ~np.isnan(ob[..., 0]).reshape(P, g - 1, g - 1)
+        t = (np.arange(res // 8) * 8 + 4).astype(np.float64)
+        tx, ty = np.meshgrid(t, t)
+
+        def side(a, b):
+            return ((b[..., 0, None] - a[..., 0, None]) * (ty.ravel() - a[..., 1, None]) -
+                    (b[..., 1, None] - a[..., 1, None]) * (tx.ravel() - a[..., 0, None]))
+
+        def inside(a, b, d):
+            s1, s2, s3 = side(a, b), side(b, d), side(d, a)
+            return ((s1 >= 0) & (s2 >= 0) & (s3 >= 0)) | ((s1 <= 0) & (s2 <= 0) & (s3 <= 0))
+
+        cov = (inside(c[0][..., 0, :], c[1][..., 0, :], c[2][..., 0, :]) |
+               inside(c[2][..., 0, :], c[3][..., 0, :], c[0][..., 0, :])).sum(-1)
+        assert np.max(np.where(valid, cov, 0)) > 1024
     assert _nan_equal(gb, ob)
     assert np.array_equal(gi, oi)
     for k, name in ((0, 'pos'), (2, 'uv')):
