@@ -147,43 +147,60 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const uint32_t* __res
     hist[(uint64_t)threadIdx.x * nblocks + blockIdx.x] = s_hist[threadIdx.x];
 }
 
+// Scatter of one 4096-item chunk: items are first ranked (stable: by digit,
+// then by position in the chunk) into LDS in digit order, then written out
+// from there, so consecutive threads write consecutive addresses of one
+// digit's run and the stores coalesce.  The chunk's digit counts come from
+// k_radix_hist (the same items, so the same counts).
 __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
     const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
     uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
     const uint64_t* __restrict__ n_ptr, int shift, int nblocks,
-    const uint64_t* __restrict__ digit_offs) {
-    __shared__ uint32_t s_run[256];
+    const uint32_t* __restrict__ hist, const uint64_t* __restrict__ digit_offs) {
+    __shared__ uint32_t s_keys[RS_CHUNK], s_vals[RS_CHUNK];
+    __shared__ uint32_t s_run[256];  // items of each digit placed so far
+    __shared__ uint32_t s_loc[256];  // first LDS slot of each digit
     __shared__ uint32_t s_wcnt[RS_THREADS / 64][256];
     const uint64_t n = *n_ptr;
     const uint64_t base = (uint64_t)blockIdx.x * RS_CHUNK;
     if (base >= n) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    // exclusive scan of the chunk's digit counts (thread t owns digit t)
+    const uint32_t cnt = hist[(uint64_t)threadIdx.x * nblocks + blockIdx.x];
+    s_loc[threadIdx.x] = cnt;
     s_run[threadIdx.x] = 0;
 #pragma unroll
     for (int w = 0; w < RS_THREADS / 64; w++) s_wcnt[w][threadIdx.x] = 0;
     __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const uint32_t v = threadIdx.x >= (unsigned)off ? s_loc[threadIdx.x - off] : 0u;
+        __syncthreads();
+        s_loc[threadIdx.x] += v;
+        __syncthreads();
+    }
+    s_loc[threadIdx.x] -= cnt;
+    __syncthreads();
     for (int r = 0; r < RS_ITEMS; r++) {
         const uint64_t idx = base + (uint64_t)r * RS_THREADS + threadIdx.x;
         const bool valid = idx < n;
-        uint32_t key = valid ? keys_in[idx] : 0u;
-        uint32_t val = valid ? vals_in[idx] : 0u;
-        uint32_t d = (key >> shift) & 255u;
+        const uint32_t key = valid ? keys_in[idx] : 0u;
+        const uint32_t val = valid ? vals_in[idx] : 0u;
+        const uint32_t d = (key >> shift) & 255u;
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; b++) {
-            uint64_t m = __ballot(valid && ((d >> b) & 1u));
+            const uint64_t m = __ballot(valid && ((d >> b) & 1u));
             peers &= ((d >> b) & 1u) ? m : ~m;
         }
         const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
         if (valid && rank == 0) s_wcnt[wave][d] = (uint32_t)__popcll(peers);
         __syncthreads();
         if (valid) {
-            uint32_t pos = s_run[d] + rank;
+            uint32_t pos = s_loc[d] + s_run[d] + rank;
             for (int w = 0; w < wave; w++) pos += s_wcnt[w][d];
-            uint64_t dst = digit_offs[(uint64_t)d * nblocks + blockIdx.x] + pos;
-            keys_out[dst] = key;
-            vals_out[dst] = val;
+            s_keys[pos] = key;
+            s_vals[pos] = val;
         }
         __syncthreads();
         uint32_t add = 0;
@@ -194,6 +211,14 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
         }
         s_run[threadIdx.x] += add;
         __syncthreads();
+    }
+    const uint32_t items = (uint32_t)min((uint64_t)RS_CHUNK, n - base);
+    for (uint32_t i = threadIdx.x; i < items; i += RS_THREADS) {
+        const uint32_t key = s_keys[i];
+        const uint32_t d = (key >> shift) & 255u;
+        const uint64_t dst = digit_offs[(uint64_t)d * nblocks + blockIdx.x] + (i - s_loc[d]);
+        keys_out[dst] = key;
+        vals_out[dst] = s_vals[i];
     }
 }
 
@@ -238,7 +263,7 @@ int radix_sort_pairs(uint32_t* keys, uint32_t* vals, const uint64_t* n_dev, uint
         int rc = scan_exclusive_u32(hist, offs, nullptr, nh, nullptr, scan_ws, s);
         if (rc) return rc;
         hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(RS_THREADS), 0, s, ka, va, kb, vb, n_dev,
-                           shift, nb, offs);
+                           shift, nb, hist, offs);
         RF_LAUNCH_CHECK("radix_sort_pairs");
         uint32_t* t = ka;
         ka = kb;
