@@ -115,14 +115,24 @@ __device__ __forceinline__ Loc loc_quad(const Geom& G, long gq) {
 struct Header {
     uint64_t required;  // (tile, quad-step) pairs the frame needs
     uint64_t n_sort;    // pairs binned (0 on overflow)
-    uint32_t overflow;  // required > capacity: nothing was rastered
-    uint32_t heavy;     // quad steps left to k_expand_heavy (more than HEAVY_PAIRS pairs)
+    uint32_t overflow;     // required > capacity: nothing was rastered
+    uint32_t heavy;        // quad steps left to k_expand_heavy
+    uint32_t setup_heavy;  // quad steps whose pairs k_count_heavy counts
+    uint32_t pad_;
 };
 
 // Setup.flags bit 5: every vertex coordinate lies within +-SMALL_COORD
 // samples, so the coverage test may use 24-bit multiplies (intersect_quad_small).
 static constexpr int FLAG_SMALL = 32;
 static constexpr int SMALL_COORD = 1 << 21;
+
+// Quad steps above HEAVY_ROWS tile rows or HEAVY_PAIRS pairs are counted and
+// expanded by whole workgroups (k_count_heavy, k_expand_heavy): a few quad
+// steps span hundreds of rows or thousands of tiles, and one lane walking
+// them would hold its wave for as many iterations.
+static constexpr uint32_t HEAVY_ROWS = 16;
+static constexpr uint32_t HEAVY_PAIRS = 1024;
+static constexpr int HEAVY_BLOCKS = 1024;  // persistent grids of the two kernels
 
 struct alignas(16) Setup {
     int vx[4], vy[4];
@@ -309,9 +319,11 @@ __global__ __launch_bounds__(256) void k_vertex(const float* __restrict__ planes
 // Step tables: blend factor and XYZ weight per (wavelength pair, sub step),
 // accumulated in float like the rasterizer's loop variables
 // (rasterizing.cl:556-557, 616-617, 644, 650-651).
-__global__ void k_steps(const float* __restrict__ spectrum, Geom G, float4* __restrict__ table) {
+__global__ void k_steps(const float* __restrict__ spectrum, Geom G, float4* __restrict__ table,
+                        Header* __restrict__ h) {
 #pragma clang fp contract(off)
     const int i = threadIdx.x + blockIdx.x * blockDim.x;
+    if (i == 0) h->setup_heavy = 0;
     if (i >= G.maxw) return;
     const float sub_step = 1.0f / (float)G.Wsub;
     const float w_step = sub_step / (float)G.W;
@@ -417,7 +429,9 @@ __global__ __launch_bounds__(256) void k_setup(const float4* __restrict__ vtx,
                                                const float4* __restrict__ table, Geom G,
                                                Setup* __restrict__ setup, Attr* __restrict__ attr,
                                                Edges* __restrict__ edges,
-                                               uint32_t* __restrict__ counts) {
+                                               uint32_t* __restrict__ counts,
+                                               uint32_t* __restrict__ heavy,
+                                               Header* __restrict__ h) {
 #pragma clang fp contract(off)
     const long e = (long)blockIdx.x * 256 + threadIdx.x;
     if (e >= G.NE) return;
@@ -508,16 +522,21 @@ __global__ __launch_bounds__(256) void k_setup(const float4* __restrict__ vtx,
                 const uint32_t ty0 = (uint32_t)(y0 / TILE), ty1 = (uint32_t)(y1 / TILE);
                 rec.r0 = tx0 | (ty0 << 16);
                 rec.r1 = tx1 | (ty1 << 16);
-                HullRows hull;
-                hull_prepare(vx, vy, &hull);
-                for (uint32_t ty = ty0; ty <= ty1; ty++) {
-                    long a, bb;
-                    hull_row_span(hull, G.AA, (int)ty, &a, &bb);
-                    a = max(a, (long)tx0);
-                    bb = min(bb, (long)tx1);
-                    if (a <= bb) count += (uint32_t)(bb - a + 1);
+                const bool tall = ty1 - ty0 + 1 > HEAVY_ROWS;
+                if (tall) {
+                    heavy[atomicAdd(&h->setup_heavy, 1u)] = (uint32_t)e;  // counted by k_count_heavy
+                } else {
+                    HullRows hull;
+                    hull_prepare(vx, vy, &hull);
+                    for (uint32_t ty = ty0; ty <= ty1; ty++) {
+                        long a, bb;
+                        hull_row_span(hull, G.AA, (int)ty, &a, &bb);
+                        a = max(a, (long)tx0);
+                        bb = min(bb, (long)tx1);
+                        if (a <= bb) count += (uint32_t)(bb - a + 1);
+                    }
                 }
-                if (count) {
+                if (count || tall) {
                     // edge rows 01, 12, 23, 30, 20 for the rasterizer
                     Edges er;
 #pragma unroll
@@ -547,11 +566,6 @@ __global__ void k_finalize(Header* __restrict__ h, uint64_t capacity) {
     h->heavy = 0;
 }
 
-// Pairs of one quad step above which its expansion is left to a whole
-// workgroup (k_expand_heavy): a few quad steps cover thousands of tiles, and
-// one lane walking them would hold its wave for thousands of iterations.
-static constexpr uint32_t HEAVY_PAIRS = 1024;
-static constexpr int HEAVY_BLOCKS = 1024;  // persistent grid of k_expand_heavy
 
 // One thread per (quad, step): walks the tile rows of its rectangle with the
 // hull span k_setup counted with.  A block's 256 entries own one contiguous
@@ -598,13 +612,16 @@ __global__ __launch_bounds__(256) void k_expand(const Setup* __restrict__ setup,
     const uint64_t end = offsets[last] + counts[last];
     const bool staged = end - begin <= EXPAND_STAGE;  // block-uniform
     const uint32_t cnt = e <= last ? counts[e] : 0u;
-    if (cnt > HEAVY_PAIRS) {
-        // its run is written by k_expand_heavy, after the staged copy below
-        heavy[atomicAdd(&h->heavy, 1u)] = (uint32_t)e;
-    } else if (cnt) {
-        uint32_t* kp = staged ? s_keys : keys;  // generic pointers: LDS or global
-        uint32_t* vp = staged ? s_vals : vals;
-        expand_entry(setup[e], G, (uint32_t)e, offsets[e] - (staged ? begin : 0), kp, vp);
+    if (cnt) {
+        const Setup rec = setup[e];
+        if (cnt > HEAVY_PAIRS || (rec.r1 >> 16) - (rec.r0 >> 16) + 1 > HEAVY_ROWS) {
+            // its run is written by k_expand_heavy, after the staged copy below
+            heavy[atomicAdd(&h->heavy, 1u)] = (uint32_t)e;
+        } else {
+            uint32_t* kp = staged ? s_keys : keys;  // generic pointers: LDS or global
+            uint32_t* vp = staged ? s_vals : vals;
+            expand_entry(rec, G, (uint32_t)e, offsets[e] - (staged ? begin : 0), kp, vp);
+        }
     }
     if (!staged) return;
     __syncthreads();
@@ -615,39 +632,102 @@ __global__ __launch_bounds__(256) void k_expand(const Setup* __restrict__ setup,
     }
 }
 
-// The quad steps k_expand deferred, one workgroup at a time: rows in order,
-// each row's span computed as k_setup computed it (every thread redundantly),
-// its tiles written by consecutive threads.  The output positions are fixed by
-// the scan, so the order the list was filled in does not matter.
+// Tile-row spans of rows [ty0 + c, ty0 + c + 256) of a quad step, one row per
+// thread, into s_a / s_len (an empty row has length 0), with s_pre the
+// exclusive prefix of the lengths; returns the chunk's pair count.
+__device__ __forceinline__ uint32_t heavy_rows(const Setup& rec, const Geom& G, uint32_t c,
+                                               uint32_t* s_a, uint32_t* s_len, uint32_t* s_pre) {
+    const uint32_t tx0 = rec.r0 & 0xffff, ty0 = rec.r0 >> 16;
+    const uint32_t tx1 = rec.r1 & 0xffff, ty1 = rec.r1 >> 16;
+    const uint32_t ty = ty0 + c + threadIdx.x;
+    uint32_t len = 0, a32 = 0;
+    if (ty <= ty1) {
+        HullRows hull;
+        hull_prepare(rec.vx, rec.vy, &hull);
+        long a, b;
+        hull_row_span(hull, G.AA, (int)ty, &a, &b);
+        a = max(a, (long)tx0);
+        b = min(b, (long)tx1);
+        if (a <= b) {
+            len = (uint32_t)(b - a + 1);
+            a32 = (uint32_t)a;
+        }
+    }
+    s_a[threadIdx.x] = a32;
+    s_len[threadIdx.x] = len;
+    s_pre[threadIdx.x] = len;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const uint32_t v = threadIdx.x >= (unsigned)off ? s_pre[threadIdx.x - off] : 0u;
+        __syncthreads();
+        s_pre[threadIdx.x] += v;
+        __syncthreads();
+    }
+    const uint32_t total = s_pre[255] + 0u;
+    __syncthreads();  // every thread has read the total before thread 255 changes it
+    s_pre[threadIdx.x] -= len;  // exclusive
+    __syncthreads();
+    return total;
+}
+
+// Pair counts of the quad steps k_setup listed (more than HEAVY_ROWS rows):
+// rows across threads, summed in the order k_setup would add them.
+__global__ __launch_bounds__(256) void k_count_heavy(const Setup* __restrict__ setup,
+                                                     const Header* __restrict__ h, Geom G,
+                                                     const uint32_t* __restrict__ heavy,
+                                                     uint32_t* __restrict__ counts) {
+    __shared__ uint32_t s_a[256], s_len[256], s_pre[256];
+    const uint32_t n = h->setup_heavy;
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+        const uint32_t e = heavy[i];
+        const Setup rec = setup[e];
+        const uint32_t rows = (rec.r1 >> 16) - (rec.r0 >> 16) + 1;
+        uint32_t count = 0;
+        for (uint32_t c = 0; c < rows; c += 256) count += heavy_rows(rec, G, c, s_a, s_len, s_pre);
+        if (threadIdx.x == 0) counts[e] = count;
+        __syncthreads();
+    }
+}
+
+// The quad steps k_expand deferred, one workgroup at a time: 256 rows at a
+// time get their spans (one row per thread) and a prefix over their lengths,
+// then the threads write the chunk's pairs, pair j finding its row by binary
+// search over the prefix.  Output positions are fixed by the scan, so the
+// order the list was filled in does not matter.
 __global__ __launch_bounds__(256) void k_expand_heavy(const Setup* __restrict__ setup,
                                                       const uint64_t* __restrict__ offsets,
                                                       const Header* __restrict__ h, Geom G,
                                                       const uint32_t* __restrict__ heavy,
                                                       uint32_t* __restrict__ keys,
                                                       uint32_t* __restrict__ vals) {
+    __shared__ uint32_t s_a[256], s_len[256], s_pre[256];
     if (h->overflow) return;
     const uint32_t n = h->heavy;
     for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
         const uint32_t e = heavy[i];
         const Setup rec = setup[e];
         uint64_t off = offsets[e];
-        const uint32_t tx0 = rec.r0 & 0xffff, ty0 = rec.r0 >> 16;
-        const uint32_t tx1 = rec.r1 & 0xffff, ty1 = rec.r1 >> 16;
-        HullRows hull;
-        hull_prepare(rec.vx, rec.vy, &hull);
+        const uint32_t ty0 = rec.r0 >> 16, rows = (rec.r1 >> 16) - ty0 + 1;
         const uint32_t frame_key =
             G.batch ? (uint32_t)loc_quad(G, rec.prim).frame * (uint32_t)(G.ntx * G.nty) : 0u;
-        for (uint32_t ty = ty0; ty <= ty1; ty++) {
-            long a, b;
-            hull_row_span(hull, G.AA, (int)ty, &a, &b);
-            a = max(a, (long)tx0);
-            b = min(b, (long)tx1);
-            if (a > b) continue;
-            for (long tx = a + threadIdx.x; tx <= b; tx += 256) {
-                keys[off + (tx - a)] = frame_key + ty * (uint32_t)G.ntx + (uint32_t)tx;
-                vals[off + (tx - a)] = e;
+        for (uint32_t c = 0; c < rows; c += 256) {
+            const uint32_t total = heavy_rows(rec, G, c, s_a, s_len, s_pre);
+            for (uint32_t j = threadIdx.x; j < total; j += 256) {
+                int lo = 0, hi = 255;  // last row whose prefix is <= j
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (s_pre[mid] <= j) lo = mid;
+                    else hi = mid - 1;
+                }
+                // rows of length 0 share their successor's prefix; skip to a real one
+                while (lo > 0 && s_len[lo] == 0) lo--;
+                const uint32_t ty = ty0 + c + (uint32_t)lo;
+                const uint32_t tx = s_a[lo] + (j - s_pre[lo]);
+                keys[off + j] = frame_key + ty * (uint32_t)G.ntx + tx;
+                vals[off + j] = e;
             }
-            off += (uint64_t)(b - a + 1);
+            off += total;
+            __syncthreads();  // the row tables are rebuilt for the next chunk
         }
     }
 }
@@ -1160,10 +1240,13 @@ static int raster_ptrs(const Geom& G, uint64_t cap, void* ws, size_t ws_bytes, R
 static int bin_impl(const float4* vtx, const float4* bounds, const float* spectrum, const Geom& G,
                     uint64_t cap, const RasterPtrs& r, hipStream_t s) {
     const long ntiles = (long)G.ntx * G.nty * G.frames;
-    hipLaunchKernelGGL(k_steps, dim3(blocks_for(G.maxw, 64)), dim3(64), 0, s, spectrum, G, r.table);
+    hipLaunchKernelGGL(k_steps, dim3(blocks_for(G.maxw, 64)), dim3(64), 0, s, spectrum, G, r.table,
+                       r.h);
     if (G.NE > 0) {
         hipLaunchKernelGGL(k_setup, dim3(blocks_for(G.NE, 256)), dim3(256), 0, s, vtx, bounds,
-                           r.table, G, r.setup, r.attr, r.edges, r.counts);
+                           r.table, G, r.setup, r.attr, r.edges, r.counts, r.heavy, r.h);
+        hipLaunchKernelGGL(k_count_heavy, dim3(HEAVY_BLOCKS), dim3(256), 0, s, r.setup, r.h, G,
+                           r.heavy, r.counts);
     }
     int rc = scan_exclusive_u32(r.counts, r.offsets, nullptr, (uint64_t)G.NE, &r.h->required,
                                 r.scan_ws, s);
