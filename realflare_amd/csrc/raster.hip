@@ -119,6 +119,7 @@ struct Header {
     uint32_t heavy;        // quad steps left to k_expand_heavy
     uint32_t setup_heavy;  // quad steps whose pairs k_count_heavy counts
     uint32_t pad_;
+    uint64_t blocks;       // super-blocks of the run (frames x per frame), k_raster's order
 };
 
 // Setup.flags bit 5: every vertex coordinate lies within +-SMALL_COORD
@@ -904,26 +905,64 @@ __device__ __forceinline__ float ghost_sample(__amdgpu_buffer_rsrc_t tex, int w,
 // Workgroups are dealt to the XCDs round-robin (workgroup i runs on XCD
 // i % RF_XCDS) and each XCD has its own L2.  A quad's records are read by every
 // tile it overlaps, so neighbouring tiles should share an L2: the screen is cut
-// into SB x SB-tile super-blocks (SB = G.sb), super-block b goes to XCD
-// b % RF_XCDS (which keeps the XCDs balanced), and an XCD walks its
-// super-blocks in order, the SB * SB tiles of one super-block back to back.
+// into SB x SB-tile super-blocks (SB = G.sb) and the SB * SB tiles of one
+// super-block run back to back on one XCD.  Super-blocks are taken in the
+// order `order` gives (the binner sorts them by pair count, largest first, so
+// the longest tiles start early and the end of the launch is not left to a
+// few of them): the r-th super-block goes to XCD r % RF_XCDS.
 static constexpr int RF_XCDS = 8;
 
-__host__ __device__ __forceinline__ long raster_grid(const Geom& G) {
+__host__ __device__ __forceinline__ long super_blocks(const Geom& G) {
     const int SB = G.sb;
-    const long nsb = (long)((G.ntx + SB - 1) / SB) * ((G.nty + SB - 1) / SB);
-    const long per_xcd = (nsb + RF_XCDS - 1) / RF_XCDS;
-    return per_xcd * RF_XCDS * SB * SB;
+    return (long)((G.ntx + SB - 1) / SB) * ((G.nty + SB - 1) / SB);
 }
 
-__device__ __forceinline__ bool raster_tile(unsigned bid, const Geom& G, int* tx, int* ty) {
+__host__ __device__ __forceinline__ long raster_grid(const Geom& G) {
+    const long per_xcd = (super_blocks(G) * G.frames + RF_XCDS - 1) / RF_XCDS;
+    return per_xcd * RF_XCDS * G.sb * G.sb;
+}
+
+__device__ __forceinline__ bool raster_tile(unsigned bid, const Geom& G,
+                                            const uint32_t* __restrict__ order, int* frame,
+                                            int* tx, int* ty) {
     const int SB = G.sb;
     const unsigned xcd = bid % RF_XCDS, j = bid / RF_XCDS;
-    const unsigned sb = (j / (SB * SB)) * RF_XCDS + xcd, t = j % (SB * SB);
+    const unsigned rank = (j / (SB * SB)) * RF_XCDS + xcd, t = j % (SB * SB);
+    const long nsb = super_blocks(G);
+    if ((long)rank >= nsb * G.frames) return false;
+    const unsigned v = order[rank];
     const int nsbx = (G.ntx + SB - 1) / SB;
+    const unsigned sb = v % (unsigned)nsb;
+    *frame = (int)(v / (unsigned)nsb);
     *tx = (int)(sb % nsbx) * SB + (int)(t % SB);
     *ty = (int)(sb / nsbx) * SB + (int)(t / SB);
     return *tx < G.ntx && *ty < G.nty;
+}
+
+// Cost of every super-block of the run (its tiles' pair counts, in units of 4
+// and capped at 2^16 - 1) as a sort key that puts the largest first; value =
+// frame * super_blocks + super-block.
+__global__ __launch_bounds__(256) void k_block_cost(const uint32_t* __restrict__ tstart,
+                                                    const uint32_t* __restrict__ tend, Geom G,
+                                                    uint32_t* __restrict__ keys,
+                                                    uint32_t* __restrict__ vals,
+                                                    Header* __restrict__ h) {
+    const long nsb = super_blocks(G);
+    const long n = nsb * G.frames;
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i == 0) h->blocks = (uint64_t)n;
+    if (i >= n) return;
+    const int SB = G.sb, nsbx = (G.ntx + SB - 1) / SB;
+    const long frame = i / nsb, sb = i % nsb;
+    const int x0 = (int)(sb % nsbx) * SB, y0 = (int)(sb / nsbx) * SB;
+    uint64_t total = 0;
+    for (int ty = y0; ty < min(y0 + SB, G.nty); ty++)
+        for (int tx = x0; tx < min(x0 + SB, G.ntx); tx++) {
+            const long t = frame * G.ntx * G.nty + (long)ty * G.ntx + tx;
+            total += tend[t] - tstart[t];
+        }
+    keys[i] = 0xffffu - (uint32_t)min(total >> 2, (uint64_t)0xffffu);  // 16-bit key: 2 passes
+    vals[i] = (uint32_t)i;
 }
 
 // fragment_shader (rasterizing.cl:471-506) for one covered pixel of one quad
@@ -982,6 +1021,7 @@ __global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ bounds
                                                const uint32_t* __restrict__ vals,
                                                const uint32_t* __restrict__ tstart,
                                                const uint32_t* __restrict__ tend,
+                                               const uint32_t* __restrict__ order,
                                                const Header* __restrict__ h, Geom G,
                                                float4* __restrict__ image) {
 #pragma clang fp contract(off)
@@ -989,10 +1029,8 @@ __global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ bounds
     __shared__ uint4 s_slot[64];  // per queued pair: coverage mask lo, hi, first queue index, entry
     __shared__ float4 s_c[64];    // shaded colour of each queue entry
     if (h->overflow) return;
-    const long per_frame = raster_grid(G);
-    const int frame = (int)(blockIdx.x / per_frame);
-    int tx, ty;
-    if (!raster_tile((unsigned)(blockIdx.x - frame * per_frame), G, &tx, &ty)) return;
+    int frame, tx, ty;
+    if (!raster_tile(blockIdx.x, G, order, &frame, &tx, &ty)) return;
     const int tile = frame * G.ntx * G.nty + ty * G.ntx + tx;
     const int lane = threadIdx.x;
     const int x = tx * TILE + (lane & (TILE - 1)), y = ty * TILE + (lane / TILE);
@@ -1169,8 +1207,8 @@ static int make_geom(const rf_raster_params* p, Geom* G) {
 static inline unsigned blocks_for(long n, int threads) { return (unsigned)((n + threads - 1) / threads); }
 
 struct Layout {
-    size_t header, table, setup, attr, edges, heavy, counts, offsets, scan_ws, keys, vals, radix_ws, tstart,
-        tend, total;
+    size_t header, table, setup, attr, edges, heavy, order_keys, order_vals, counts, offsets,
+        scan_ws, keys, vals, radix_ws, tstart, tend, total;
 };
 
 static Layout layout(const Geom& G, uint64_t cap) {
@@ -1187,12 +1225,15 @@ static Layout layout(const Geom& G, uint64_t cap) {
     L.attr = take((size_t)G.NE * sizeof(Attr));
     L.edges = take((size_t)G.NE * sizeof(Edges));
     L.heavy = take((size_t)G.NE * sizeof(uint32_t));
+    const uint64_t nblk = (uint64_t)(super_blocks(G) * G.frames);
+    L.order_keys = take(nblk * sizeof(uint32_t));
+    L.order_vals = take(nblk * sizeof(uint32_t));
     L.counts = take((size_t)G.NE * sizeof(uint32_t));
     L.offsets = take((size_t)G.NE * sizeof(uint64_t));
     L.scan_ws = take(scan_workspace_bytes((uint64_t)G.NE));
     L.keys = take(cap * sizeof(uint32_t));
     L.vals = take(cap * sizeof(uint32_t));
-    L.radix_ws = take(radix_workspace_bytes(cap));
+    L.radix_ws = take(radix_workspace_bytes(cap > nblk ? cap : nblk));
     const size_t ntiles = (size_t)G.ntx * G.nty * G.frames;
     L.tstart = take(ntiles * sizeof(uint32_t));
     L.tend = take(ntiles * sizeof(uint32_t));
@@ -1207,6 +1248,7 @@ struct RasterPtrs {
     Attr* attr;
     Edges* edges;
     uint32_t* heavy;
+    uint32_t *order_keys, *order_vals;
     uint32_t *counts, *keys, *vals, *tstart, *tend;
     uint64_t* offsets;
     void *scan_ws, *radix_ws;
@@ -1225,6 +1267,8 @@ static int raster_ptrs(const Geom& G, uint64_t cap, void* ws, size_t ws_bytes, R
     r->attr = (Attr*)(w + L.attr);
     r->edges = (Edges*)(w + L.edges);
     r->heavy = (uint32_t*)(w + L.heavy);
+    r->order_keys = (uint32_t*)(w + L.order_keys);
+    r->order_vals = (uint32_t*)(w + L.order_vals);
     r->counts = (uint32_t*)(w + L.counts);
     r->offsets = (uint64_t*)(w + L.offsets);
     r->scan_ws = w + L.scan_ws;
@@ -1269,6 +1313,13 @@ static int bin_impl(const float4* vtx, const float4* bounds, const float* spectr
         hipLaunchKernelGGL(k_ranges, dim3(blocks_for((long)cap, 256)), dim3(256), 0, s, r.keys, r.h,
                            r.tstart, r.tend);
     }
+    // the rasterizer's super-block order: largest pair count first (stable)
+    const long nblk = super_blocks(G) * G.frames;
+    hipLaunchKernelGGL(k_block_cost, dim3(blocks_for(nblk, 256)), dim3(256), 0, s, r.tstart, r.tend,
+                       G, r.order_keys, r.order_vals, r.h);
+    rc = radix_sort_pairs(r.order_keys, r.order_vals, &r.h->blocks, (uint64_t)nblk, 16, r.radix_ws,
+                          s);
+    if (rc) return rc;
     RF_LAUNCH_CHECK("binner");
     return RF_OK;
 }
@@ -1278,9 +1329,9 @@ static int draw_impl(const float4* vtx, const float4* bounds, const float* ghost
     const long ntiles = (long)G.ntx * G.nty;
     (void)vtx;
     (void)ntiles;
-    hipLaunchKernelGGL(k_raster, dim3((unsigned)(raster_grid(G) * G.frames)), dim3(64), 0, s, bounds,
-                       ghost, r.setup, r.edges,
-                       r.attr, r.vals, r.tstart, r.tend, r.h, G, image);
+    hipLaunchKernelGGL(k_raster, dim3((unsigned)raster_grid(G)), dim3(64), 0, s, bounds, ghost,
+                       r.setup, r.edges, r.attr, r.vals, r.tstart, r.tend, r.order_vals, r.h, G,
+                       image);
     RF_LAUNCH_CHECK("rasterizer");
     return RF_OK;
 }
