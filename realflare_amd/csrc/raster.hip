@@ -633,14 +633,24 @@ __global__ __launch_bounds__(256) void k_expand(const Setup* __restrict__ setup,
     }
 }
 
-// Tile-row spans of rows [ty0 + c, ty0 + c + 256) of a quad step, one row per
-// thread, into s_a / s_len (an empty row has length 0), with s_pre the
-// exclusive prefix of the lengths; returns the chunk's pair count.
-__device__ __forceinline__ uint32_t heavy_rows(const Setup& rec, const Geom& G, uint32_t c,
+// Both kernels below give each listed quad step to one wave (four per
+// workgroup, independent of each other, so they synchronize at wave level
+// only): 64 rows at a time, one row per lane.
+static constexpr int HW_WAVES = 4;
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Tile-row span of row ty0 + c + lane of a quad step (an empty row or a row
+// past the last has length 0) and the exclusive prefix of the lengths over
+// the wave, in s_a / s_len / s_pre; returns the 64 rows' pair count.
+__device__ __forceinline__ uint32_t heavy_rows(const Setup& rec, const Geom& G, uint32_t c, int lane,
                                                uint32_t* s_a, uint32_t* s_len, uint32_t* s_pre) {
     const uint32_t tx0 = rec.r0 & 0xffff, ty0 = rec.r0 >> 16;
     const uint32_t tx1 = rec.r1 & 0xffff, ty1 = rec.r1 >> 16;
-    const uint32_t ty = ty0 + c + threadIdx.x;
+    const uint32_t ty = ty0 + c + (uint32_t)lane;
     uint32_t len = 0, a32 = 0;
     if (ty <= ty1) {
         HullRows hull;
@@ -654,45 +664,42 @@ __device__ __forceinline__ uint32_t heavy_rows(const Setup& rec, const Geom& G, 
             a32 = (uint32_t)a;
         }
     }
-    s_a[threadIdx.x] = a32;
-    s_len[threadIdx.x] = len;
-    s_pre[threadIdx.x] = len;
-    __syncthreads();
-    for (int off = 1; off < 256; off <<= 1) {
-        const uint32_t v = threadIdx.x >= (unsigned)off ? s_pre[threadIdx.x - off] : 0u;
-        __syncthreads();
-        s_pre[threadIdx.x] += v;
-        __syncthreads();
+    uint32_t inc = len;  // inclusive scan over the wave
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t v = (uint32_t)__shfl_up((int)inc, off, 64);
+        if (lane >= off) inc += v;
     }
-    const uint32_t total = s_pre[255] + 0u;
-    __syncthreads();  // every thread has read the total before thread 255 changes it
-    s_pre[threadIdx.x] -= len;  // exclusive
-    __syncthreads();
+    const uint32_t total = (uint32_t)__shfl((int)inc, 63, 64);
+    wave_sync();  // the previous chunk's tables have been read
+    s_a[lane] = a32;
+    s_len[lane] = len;
+    s_pre[lane] = inc - len;
+    wave_sync();
     return total;
 }
 
-// Pair counts of the quad steps k_setup listed (more than HEAVY_ROWS rows):
-// rows across threads, summed in the order k_setup would add them.
+// Pair counts of the quad steps k_setup listed (more than HEAVY_ROWS rows).
 __global__ __launch_bounds__(256) void k_count_heavy(const Setup* __restrict__ setup,
                                                      const Header* __restrict__ h, Geom G,
                                                      const uint32_t* __restrict__ heavy,
                                                      uint32_t* __restrict__ counts) {
-    __shared__ uint32_t s_a[256], s_len[256], s_pre[256];
+    __shared__ uint32_t s_a[HW_WAVES][64], s_len[HW_WAVES][64], s_pre[HW_WAVES][64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t n = h->setup_heavy;
-    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+    for (uint32_t i = blockIdx.x * HW_WAVES + wave; i < n; i += gridDim.x * HW_WAVES) {
         const uint32_t e = heavy[i];
         const Setup rec = setup[e];
         const uint32_t rows = (rec.r1 >> 16) - (rec.r0 >> 16) + 1;
         uint32_t count = 0;
-        for (uint32_t c = 0; c < rows; c += 256) count += heavy_rows(rec, G, c, s_a, s_len, s_pre);
-        if (threadIdx.x == 0) counts[e] = count;
-        __syncthreads();
+        for (uint32_t c = 0; c < rows; c += 64)
+            count += heavy_rows(rec, G, c, lane, s_a[wave], s_len[wave], s_pre[wave]);
+        if (lane == 0) counts[e] = count;
     }
 }
 
-// The quad steps k_expand deferred, one workgroup at a time: 256 rows at a
-// time get their spans (one row per thread) and a prefix over their lengths,
-// then the threads write the chunk's pairs, pair j finding its row by binary
+// The quad steps k_expand deferred: per 64 rows, the rows' spans and prefix,
+// then the lanes write the chunk's pairs, pair j finding its row by binary
 // search over the prefix.  Output positions are fixed by the scan, so the
 // order the list was filled in does not matter.
 __global__ __launch_bounds__(256) void k_expand_heavy(const Setup* __restrict__ setup,
@@ -701,34 +708,37 @@ __global__ __launch_bounds__(256) void k_expand_heavy(const Setup* __restrict__ 
                                                       const uint32_t* __restrict__ heavy,
                                                       uint32_t* __restrict__ keys,
                                                       uint32_t* __restrict__ vals) {
-    __shared__ uint32_t s_a[256], s_len[256], s_pre[256];
+    __shared__ uint32_t s_a[HW_WAVES][64], s_len[HW_WAVES][64], s_pre[HW_WAVES][64];
     if (h->overflow) return;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t n = h->heavy;
-    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+    for (uint32_t i = blockIdx.x * HW_WAVES + wave; i < n; i += gridDim.x * HW_WAVES) {
         const uint32_t e = heavy[i];
         const Setup rec = setup[e];
         uint64_t off = offsets[e];
         const uint32_t ty0 = rec.r0 >> 16, rows = (rec.r1 >> 16) - ty0 + 1;
         const uint32_t frame_key =
             G.batch ? (uint32_t)loc_quad(G, rec.prim).frame * (uint32_t)(G.ntx * G.nty) : 0u;
-        for (uint32_t c = 0; c < rows; c += 256) {
-            const uint32_t total = heavy_rows(rec, G, c, s_a, s_len, s_pre);
-            for (uint32_t j = threadIdx.x; j < total; j += 256) {
-                int lo = 0, hi = 255;  // last row whose prefix is <= j
+        const uint32_t* pa = s_a[wave];
+        const uint32_t* pl = s_len[wave];
+        const uint32_t* pp = s_pre[wave];
+        for (uint32_t c = 0; c < rows; c += 64) {
+            const uint32_t total = heavy_rows(rec, G, c, lane, s_a[wave], s_len[wave], s_pre[wave]);
+            for (uint32_t j = (uint32_t)lane; j < total; j += 64) {
+                int lo = 0, hi = 63;  // last row whose prefix is <= j
                 while (lo < hi) {
                     const int mid = (lo + hi + 1) >> 1;
-                    if (s_pre[mid] <= j) lo = mid;
+                    if (pp[mid] <= j) lo = mid;
                     else hi = mid - 1;
                 }
-                // rows of length 0 share their successor's prefix; skip to a real one
-                while (lo > 0 && s_len[lo] == 0) lo--;
+                // rows of length 0 share their successor's prefix; step back to a real one
+                while (lo > 0 && pl[lo] == 0) lo--;
                 const uint32_t ty = ty0 + c + (uint32_t)lo;
-                const uint32_t tx = s_a[lo] + (j - s_pre[lo]);
+                const uint32_t tx = pa[lo] + (j - pp[lo]);
                 keys[off + j] = frame_key + ty * (uint32_t)G.ntx + tx;
                 vals[off + j] = e;
             }
             off += total;
-            __syncthreads();  // the row tables are rebuilt for the next chunk
         }
     }
 }
