@@ -55,23 +55,36 @@ __global__ __launch_bounds__(256) void k_starburst(const float* __restrict__ spe
     const int lambda_delta = 730 - 390;
     float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f, r3 = 0.0f;
     const float fx = (float)px, fy = (float)py;
+    // No blur and no rotation (the defaults): the jitter radius and the
+    // rotation angle are exactly 0, so the position is nx, ny scaled by the
+    // wavelength (cos 0 = 1 and sin 0 = 0 leave it unchanged up to the sign of
+    // a zero, which the texture coordinate 0.5 * x + 0.5 does not see), and the
+    // six noise and trigonometric evaluations per sample are skipped.
+    const bool still = p.blur == 0.0f && p.rotation == 0.0f && p.rotation_weight >= 0.0f;
     for (int t = 0; t < samples; ++t) {
         const float step = (float)t / (float)samples;
         const float wavelength = step * (float)lambda_delta + 390.0f;
-        const float seed = (float)(t * 4);
-        float x = nx, y = ny;
-        const float radius = p.blur * sqrtf(sb_noise(fx, fy, seed));
-        const float alpha = sb_noise(fx, fy, seed + 1) * 2 * pi;
-        x += cosf(alpha) * radius;
-        y += sinf(alpha) * radius;
         const float sc = fmaxf(1 - (1 - (wavelength / 560.0f)) * 1.0f, FLT_MIN);
-        x = x / sc;
-        y = y / sc;
-        const float sign = (sb_noise(fx, fy, seed + 2) > 0.5f) ? 1.0f : -1.0f;
-        const float angle =
-            sign * p.rotation * (1.0f - powf(sb_noise(fx, fy, seed + 3), p.rotation_weight));
-        const float sa = sinf(angle), ca = cosf(angle);
-        const float rx = x * ca + y * sa, ry = y * ca - x * sa;
+        float rx, ry;
+        if (still) {
+            rx = nx / sc;
+            ry = ny / sc;
+        } else {
+            const float seed = (float)(t * 4);
+            float x = nx, y = ny;
+            const float radius = p.blur * sqrtf(sb_noise(fx, fy, seed));
+            const float alpha = sb_noise(fx, fy, seed + 1) * 2 * pi;
+            x += cosf(alpha) * radius;
+            y += sinf(alpha) * radius;
+            x = x / sc;
+            y = y / sc;
+            const float sign = (sb_noise(fx, fy, seed + 2) > 0.5f) ? 1.0f : -1.0f;
+            const float angle =
+                sign * p.rotation * (1.0f - powf(sb_noise(fx, fy, seed + 3), p.rotation_weight));
+            const float sa = sinf(angle), ca = cosf(angle);
+            rx = x * ca + y * sa;
+            ry = y * ca - x * sa;
+        }
         const float u = rx * 0.5f + 0.5f, v = ry * 0.5f + 0.5f;
         const float fi = sample_bilinear(spectrum, sw, sh, 1, u, v, false);
         float4 xyz;
