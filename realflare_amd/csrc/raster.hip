@@ -1014,15 +1014,20 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
 }
 
+// Fragment queue capacity: two waves' worth, so a flush shades two passes and
+// its fixed costs (barriers, the colour exchange) are paid half as often.
+// 4.6 KB of LDS per tile keeps the occupancy the registers allow; more
+// (256 entries, 9 KB) would lower it.
+static constexpr uint32_t QCAP = 128;
 
 // rasterizer (rasterizing.cl:524-663): one wave per 8x8 tile, lane = pixel.
 // The tile's (quad step) pairs are walked in binning order.  A pair's coverage
-// is tested by every pixel lane; the covered pixels are appended to a 64-entry
+// is tested by every pixel lane; the covered pixels are appended to a QCAP-entry
 // fragment queue in LDS (pixel lane, covered samples, pair slot).  When the
-// queue is full, lane i shades fragment i, and the colours go back to their
-// pixel lanes slot by slot, i.e. in pair order, so each pixel adds exactly the
-// terms of the per-pixel loop in the same order.  Shading thus runs on full
-// waves even when a quad covers few pixels of the tile.
+// queue is full, lane i shades fragments i, i + 64, ..., and the colours go back
+// to their pixel lanes slot by slot, i.e. in pair order, so each pixel adds
+// exactly the terms of the per-pixel loop in the same order.  Shading thus runs
+// on full waves even when a quad covers few pixels of the tile.
 __global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ bounds,
                                                const float* __restrict__ ghost,
                                                const Setup* __restrict__ setup,
@@ -1035,9 +1040,9 @@ __global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ bounds
                                                const Header* __restrict__ h, Geom G,
                                                float4* __restrict__ image) {
 #pragma clang fp contract(off)
-    __shared__ uint32_t s_q[64];  // pixel lane | hits << 6 | slot << 10
-    __shared__ uint4 s_slot[64];  // per queued pair: coverage mask lo, hi, first queue index, entry
-    __shared__ float4 s_c[64];    // shaded colour of each queue entry
+    __shared__ uint32_t s_q[QCAP];  // pixel lane | hits << 6 | slot << 10
+    __shared__ uint4 s_slot[QCAP];  // per queued pair: coverage mask lo, hi, first queue index, entry
+    __shared__ float4 s_c[QCAP];    // shaded colour of each queue entry
     if (h->overflow) return;
     int frame, tx, ty;
     if (!raster_tile(blockIdx.x, G, order, &frame, &tx, &ty)) return;
@@ -1056,21 +1061,24 @@ __global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ bounds
 
     auto flush = [&]() {
         __syncthreads();  // queue entries written by other lanes are visible
-        float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
-        if ((uint32_t)lane < qlen) {
-            const uint32_t ent = s_q[lane];
-            const int p = ent & 63, hits = (ent >> 6) & 15;
-            const uint32_t e = s_slot[ent >> 10].w;
-            const Setup rec = setup[e];
-            const Attr at = attr[e];
-            const int fx = tx * TILE + (p & (TILE - 1)), fy = ty * TILE + p / TILE;
-            const float frag = shade_fragment(fx * G.AA + G.AA / 2, fy * G.AA + G.AA / 2, rec, at,
-                                              hits, ghost_rsrc, G, gmap);
-            c0 = at.x * frag;
-            c1 = at.y * frag;
-            c2 = at.z * frag;
+        for (uint32_t q0 = 0; q0 < qlen; q0 += 64) {
+            const uint32_t q = q0 + (uint32_t)lane;
+            float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
+            if (q < qlen) {
+                const uint32_t ent = s_q[q];
+                const int p = ent & 63, hits = (ent >> 6) & 15;
+                const uint32_t e = s_slot[ent >> 10].w;
+                const Setup rec = setup[e];
+                const Attr at = attr[e];
+                const int fx = tx * TILE + (p & (TILE - 1)), fy = ty * TILE + p / TILE;
+                const float frag = shade_fragment(fx * G.AA + G.AA / 2, fy * G.AA + G.AA / 2, rec,
+                                                  at, hits, ghost_rsrc, G, gmap);
+                c0 = at.x * frag;
+                c1 = at.y * frag;
+                c2 = at.z * frag;
+            }
+            s_c[q] = make_float4(c0, c1, c2, 0.0f);
         }
-        s_c[lane] = make_float4(c0, c1, c2, 0.0f);
         __syncthreads();  // colours visible to their pixel lanes
         for (uint32_t sl = 0; sl < nslot; sl++) {
             const uint4 si = s_slot[sl];
@@ -1121,7 +1129,7 @@ __global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ bounds
         const uint64_t m = __ballot(hits > 0);
         if (m == 0) continue;
         const uint32_t n = (uint32_t)__popcll(m);
-        if (qlen + n > 64) flush();
+        if (qlen + n > QCAP) flush();
         if (hits > 0) s_q[qlen + lanes_below(m)] = (uint32_t)lane | (uint32_t)hits << 6 | nslot << 10;
         if (lane == 0) s_slot[nslot] = make_uint4((uint32_t)m, (uint32_t)(m >> 32), qlen, e);
         qlen += n;
