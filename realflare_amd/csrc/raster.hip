@@ -796,45 +796,6 @@ __device__ __forceinline__ bool cover_small(int px, int py, const Edges& er) {
     return front || back;
 }
 
-// rasterizing.cl:95-148 (mean value coordinates of a quad)
-__device__ __forceinline__ void barycentric_quad(int px, int py, const int* x, const int* y,
-                                                 float* w) {
-#pragma clang fp contract(off)
-    float sx[4], sy[4], a[4], d[4], r[4];
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        sx[i] = (float)wsub(x[i], px);
-        sy[i] = (float)wsub(y[i], py);
-    }
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        const int j = (i + 1) & 3;
-        a[i] = sx[i] * sy[j] - sx[j] * sy[i];
-        d[i] = sx[i] * sx[j] + sy[i] * sy[j];
-        r[i] = sqrtf(sx[i] * sx[i] + sy[i] * sy[i]);
-    }
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        if (r[i] == 0) {
-#pragma unroll
-            for (int k = 0; k < 4; k++) w[k] = (k == i) ? 1.0f : 0.0f;
-            return;
-        }
-    }
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-        if (a[i] == 0) a[i] = 1e-3f;
-    float t[4];
-#pragma unroll
-    for (int i = 0; i < 4; i++) t[i] = (r[i] * r[(i + 1) & 3] - d[i]) / a[i];
-    float ww[4];
-#pragma unroll
-    for (int i = 0; i < 4; i++) ww[i] = (t[(i + 3) & 3] + t[i]) / r[i];
-    const float sum = ww[0] + ww[1] + ww[2] + ww[3];
-#pragma unroll
-    for (int i = 0; i < 4; i++) w[i] = ww[i] / sum;
-}
-
 // rasterizing.cl:95-148 (mean value coordinates of a quad), shading precision:
 // hardware reciprocal square root, reciprocal and fused multiply-adds (the
 // weights only scale a fragment's colour; coverage is decided elsewhere in
