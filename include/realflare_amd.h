@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define RF_ABI_VERSION 1
+#define RF_ABI_VERSION 2  /* 2: rf_trace_fit takes huber_delta */
 
 typedef void *rf_stream_t;
 
@@ -213,8 +213,10 @@ int rf_trace_batch(const rf_lens_element *lenses, const int32_t *lens_table,
  *                       sum of W * ceil(g^2 / (64 * rays_per_lane)); chunk_count
  *                       its last value
  *   target_*            float[ray_total] per ray (NaN: no target)
- *   loss per ray        loss_scale * (dx^2 + dy^2 + (dr / reflectance_scale)^2),
- *                       for rays that reach the sensor in trace and target
+ *   loss per ray        loss_scale * (f(dx) + f(dy) + f(dr / reflectance_scale))
+ *                       for rays that reach the sensor in trace and target, with
+ *                       f(r) = r^2, or with huber_delta > 0 and |r| > huber_delta
+ *                       f(r) = 2 huber_delta |r| - huber_delta^2
  *   grad_out            float[instances][(3 + W) * max_elements + 1]: d/d radius,
  *                       d/d centre, d/d coating (max_elements each), d/d ior
  *                       for each wavelength, then the instance's loss
@@ -231,8 +233,8 @@ int rf_trace_fit(const rf_lens_element *lenses, const float *coatings, const int
                  int32_t lens_instances, int32_t max_elements, int32_t rays_per_lane,
                  int32_t checkpoint_every, int32_t workgroups, const float *target_x,
                  const float *target_y, const float *target_reflectance, float reflectance_scale,
-                 float loss_scale, void *workspace, size_t workspace_bytes, float *grad_out,
-                 float *forward_out, rf_stream_t stream);
+                 float loss_scale, float huber_delta, void *workspace, size_t workspace_bytes,
+                 float *grad_out, float *forward_out, rf_stream_t stream);
 
 /* Settings of the rasterizer kernels (rasterizing.cl:233-663), as
  * RasterizingTask.rasterize derives them (rasterizing.py:296-421). */

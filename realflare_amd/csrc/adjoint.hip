@@ -798,8 +798,8 @@ static AdjKernel adj_kernel(int K, int mode = ADJ_FUSED) {
 // after the other, keeping checkpoints in its own scratch column
 // ([workgroup][slot][field][64]), so scratch does not grow with the batch.
 // The loss of a ray that reaches the sensor in the trace and in the target is
-// loss_scale * (dx^2 + dy^2 + (dr / refl_scale)^2); its gradient seeds the
-// reverse pass in place.  Partials per chunk are laid out [chunk][4 LF + 1]:
+// loss_scale * (f(dx) + f(dy) + f(dr / refl_scale)), f the square or its Huber
+// form (fit_term); its gradient seeds the reverse pass in place.  Partials per chunk are laid out [chunk][4 LF + 1]:
 // radius, centre, coating, n_eff of the chunk's wavelength (each LF wide),
 // then the loss.
 
@@ -818,8 +818,23 @@ struct FitArgs {
     float* fwd_out;
     long ray_total;
     int nseg, W, LF, rpl, slots;
-    float cmin, inv_refl_scale, loss_scale;
+    float cmin, inv_refl_scale, loss_scale, huber;
 };
+
+// Per-component loss and its derivative: r^2 (derivative 2 r), or with a
+// Huber threshold delta > 0 and |r| > delta, 2 delta |r| - delta^2 (derivative
+// 2 delta sign r): large residuals (rays whose path changes between the trace
+// and the target) then pull linearly instead of quadratically.
+__device__ __forceinline__ float fit_term(float r, float delta, float* d) {
+#pragma clang fp contract(off)
+    const float ar = fabsf(r);
+    if (delta > 0.0f && ar > delta) {
+        *d = 2.0f * copysignf(delta, r);
+        return 2.0f * delta * ar - delta * delta;
+    }
+    *d = 2.0f * r;
+    return r * r;
+}
 
 template <int K>
 __global__ __launch_bounds__(ADJ_THREADS) void k_trace_fit(FitArgs a) {
@@ -887,11 +902,13 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_trace_fit(FitArgs a) {
             const float xt = a.tx[idx], yt = a.ty[idx], rt = a.tr[idx];
             if (isnan(xt) || isnan(yt) || isnan(rt)) continue;
             const float dx = s.pos.x - xt, dy = s.pos.y - yt, dr = (s.refl - rt) * a.inv_refl_scale;
-            loss += a.loss_scale * (dx * dx + dy * dy + dr * dr);
-            const float gs = 2.0f * a.loss_scale;
+            float gx, gy, gr;
+            const float lx = fit_term(dx, a.huber, &gx), ly = fit_term(dy, a.huber, &gy);
+            const float lr = fit_term(dr, a.huber, &gr);
+            loss += a.loss_scale * (lx + ly + lr);
             reverse_ray<K>(Lf, s_n, s_lacc, P, ri, s, st, scratch, ADJ_THREADS, lane, true,
-                           mk3(gs * dx, gs * dy, 0.0f), mk3(0, 0, 0),
-                           gs * dr * a.inv_refl_scale);
+                           mk3(a.loss_scale * gx, a.loss_scale * gy, 0.0f), mk3(0, 0, 0),
+                           a.loss_scale * gr * a.inv_refl_scale);
         }
         float* out = a.partial + (long)c * pstride;
         for (int b = 0; b < 4; b++) {
@@ -1124,8 +1141,9 @@ extern "C" int rf_trace_fit(const rf_lens_element* lenses, const float* coatings
                             int32_t max_elements, int32_t rays_per_lane, int32_t checkpoint_every,
                             int32_t workgroups, const float* target_x, const float* target_y,
                             const float* target_reflectance, float reflectance_scale,
-                            float loss_scale, void* workspace, size_t workspace_bytes,
-                            float* grad_out, float* forward_out, rf_stream_t stream) {
+                            float loss_scale, float huber_delta, void* workspace,
+                            size_t workspace_bytes, float* grad_out, float* forward_out,
+                            rf_stream_t stream) {
     if (!lenses || !coatings || !lens_table || !paths || !wavelengths || !batch ||
         !chunk_offsets || !lens_segment_offsets || !target_x || !target_y ||
         !target_reflectance || !grad_out || !batch->segments)
@@ -1141,6 +1159,8 @@ extern "C" int rf_trace_fit(const rf_lens_element* lenses, const float* coatings
                          checkpoint_every);
     if (!(reflectance_scale > 0))
         return set_error(RF_ERR_INVALID, "rf_trace_fit: reflectance_scale must be positive");
+    if (!(huber_delta >= 0))
+        return set_error(RF_ERR_INVALID, "rf_trace_fit: huber_delta must be >= 0");
     const FitLayout F = fit_layout(max_elements, checkpoint_every, workgroups, chunk_count);
     if (!workspace || workspace_bytes < F.total)
         return set_error(RF_ERR_WORKSPACE, "rf_trace_fit: workspace %zu < %zu", workspace_bytes,
@@ -1171,6 +1191,7 @@ extern "C" int rf_trace_fit(const rf_lens_element* lenses, const float* coatings
     a.cmin = coating_min_ior;
     a.inv_refl_scale = 1.0f / reflectance_scale;
     a.loss_scale = loss_scale;
+    a.huber = huber_delta;
     hipLaunchKernelGGL(k_queue_reset_fit, dim3(1), dim3(1), 0, s, a.head);
     const size_t lds = (size_t)4 * max_elements * ADJ_THREADS * sizeof(float);
     hipLaunchKernelGGL(kern, dim3(workgroups), dim3(ADJ_THREADS), lds, s, a);

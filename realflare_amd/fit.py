@@ -125,6 +125,7 @@ class FitBatch:
         self.targets = (nan, nan, nan)
         self.refl_scale = 1.0
         self.loss_scale = 1.0
+        self.huber_delta = 0.0  # 0: squared error; > 0: Huber beyond this residual
 
     def upload(self, params: list[LensParams]) -> None:
         recs = np.zeros(self.n_elements, lens_element_dtype)  # keeps the 64-byte records
@@ -145,7 +146,8 @@ class FitBatch:
                     p(self.wl_t), self.W, ctypes.c_float(self.cmin), ctypes.byref(self.desc),
                     p(self.chunks_t), ctypes.c_uint64(self.chunk_count), p(self.seg_off_t), self.I,
                     self.LF, self.rpl, self.K, self.workgroups, p(tx), p(ty), p(tr),
-                    ctypes.c_float(self.refl_scale), ctypes.c_float(self.loss_scale), p(self.ws),
+                    ctypes.c_float(self.refl_scale), ctypes.c_float(self.loss_scale),
+                    ctypes.c_float(self.huber_delta), p(self.ws),
                     ctypes.c_size_t(self.ws.numel()), p(self.grad), p(self.fwd), native.stream())
 
     def set_targets(self, params: list[LensParams]) -> None:
@@ -168,6 +170,20 @@ class FitBatch:
             count += int(ok.sum())
         self.refl_scale = math.sqrt(total / count) if count else 1.0
         self.loss_scale = 1.0 / max(count, 1)
+
+    def residual_quantile(self, params: list[LensParams], q: float) -> float:
+        """Quantile q of the position residuals |x - x_target|, |y - y_target|
+        over the rays that reach the sensor in both (a strided sample of at most
+        2^24 values per component)."""
+        self.upload(params)
+        self.launch()
+        vals = []
+        stride = max(1, self.rays // (1 << 24))
+        for c in range(2):
+            r = (self.fwd[c, ::stride] - self.targets[c][::stride]).abs()
+            vals.append(r[~torch.isnan(r)])
+        v = torch.cat(vals)
+        return float(torch.quantile(v.float(), q)) if v.numel() else 0.0
 
     def step(self, params: list[LensParams]):
         """Loss per instance and parameter gradients per instance."""
@@ -249,16 +265,22 @@ def make_instances(models: list[LensModel], positions, grid_range=(64, 128), sen
 
 
 def fit(batch: FitBatch, start: list[LensParams], steps: int, lr: float = 1e-4,
-        on_step=None, patience: float = 2.0) -> tuple[list[LensParams], list[dict]]:
+        on_step=None, patience: float = 2.0, robust: float = 0.0
+        ) -> tuple[list[LensParams], list[dict]]:
     """`steps` Adam steps from `start`.  The instances are independent, so
     each keeps its own best point and step size.  The loss is not convex
     (ghost rays appear and vanish at aperture edges), so a step that raises an
     instance's loss is kept unless the loss exceeds `patience` times that
     instance's best; then the instance returns to its best point with half the
     step size.  `on_step(step, params)` runs after each gradient
-    evaluation.  Returns the best parameters of every instance.  History:
+    evaluation.  With `robust` = q > 0 the loss is Huber's beyond the q-quantile
+    of the starting position residuals (rays whose ghost path changes between
+    the trace and the target then pull linearly).  Returns the best parameters
+    of every instance.  History:
     summed and per-instance losses of every evaluated point and the best
     losses so far, GPU time."""
+    if robust > 0:
+        batch.huber_delta = batch.residual_quantile(start, robust)
     opt = Adam(start, lr)
     params = list(start)
     n = len(start)

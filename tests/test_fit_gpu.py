@@ -2,7 +2,7 @@
 
 * Forward: for every (instance, light, ghost) segment the fit kernel's
   outputs equal rf_trace_planes_coated's for that lens alone (bit-exact).
-* Gradient: per instance it equals the single-lens adjoint
+* Gradient (squared and Huber loss): per instance it equals the single-lens adjoint
   (rf_trace_adjoint, itself checked against FP64 finite differences and an
   FP64 PyTorch trace) fed with the same loss gradient and summed over the
   instance's light positions, within 1e-5 relative (only the summation order
@@ -64,7 +64,13 @@ def test_fit_forward_is_the_trace():
         np.testing.assert_array_equal(got[:, alive], ref[:, alive])
 
 
-def _reference_grads(fb, inst, params, targets):
+def _huber(r, delta):
+    if delta <= 0:
+        return r * r
+    return torch.where(r.abs() > delta, 2 * delta * r.abs() - delta * delta, r * r)
+
+
+def _reference_grads(fb, inst, params, targets, delta=0.0):
     """Single-lens adjoint per (instance, light), summed over lights."""
     tx, ty, tr = (t.cpu().numpy() for t in targets)
     out = []
@@ -88,7 +94,7 @@ def _reference_grads(fb, inst, params, targets):
             dx = torch.where(m, x - xt, z)
             dy = torch.where(m, y - yt, z)
             dr = torch.where(m, (r - rt) / fb.refl_scale, z)
-            (fb.loss_scale * (dx * dx + dy * dy + dr * dr).sum()).backward()
+            (fb.loss_scale * (_huber(dx, delta) + _huber(dy, delta) + _huber(dr, delta)).sum()).backward()
             gk = torch.cat([q.grad.reshape(-1) for q in tp]).numpy()
             acc = gk if acc is None else acc + gk
         out.append(acc)
@@ -123,6 +129,35 @@ def test_fit_gradient_matches_single_lens_adjoint():
                 ga = torch.cat([t.reshape(-1) for t in a]).numpy()
                 gb = torch.cat([t.reshape(-1) for t in b]).numpy()
                 assert np.linalg.norm(ga - gb) <= 1e-5 * np.linalg.norm(ga)
+
+
+def test_fit_huber_gradient_matches_single_lens_adjoint():
+    # the Huber form of the loss (residuals beyond delta pull linearly): the
+    # fit kernel's gradient equals the single-lens adjoint fed with the
+    # gradient of the same loss, and delta = 0 is the squared loss bit for bit
+    inst = _instances(grid=16)
+    fb = FitBatch(inst, WL, RES, SENSOR, checkpoint_every=4, rays_per_lane=2, workgroups=96)
+    target = [perturbed(i.params, i.static, 0.01, seed=5 + k) for k, i in enumerate(inst)]
+    fb.set_targets(target)
+    params = [i.params for i in inst]
+    _, g_sq = fb.step(params)
+    fb.huber_delta = fb.residual_quantile(params, 0.5)
+    assert fb.huber_delta > 0
+    losses, grads = fb.step(params)
+    ref = _reference_grads(fb, inst, params, fb.targets, delta=fb.huber_delta)
+    for g, r, q in zip(grads, ref, g_sq):
+        g = torch.cat([t.reshape(-1) for t in g]).numpy()
+        q = torch.cat([t.reshape(-1) for t in q]).numpy()
+        err = np.linalg.norm(g - r) / np.linalg.norm(r)
+        print(f'huber delta {fb.huber_delta:.3e}: rel err {err:.2e}, '
+              f'differs from squared by {np.linalg.norm(g - q) / np.linalg.norm(q):.2e}', flush=True)
+        assert err <= 1e-5
+        assert np.linalg.norm(g - q) > 1e-3 * np.linalg.norm(q)
+    fb.huber_delta = 0.0
+    _, g0 = fb.step(params)
+    for a, b in zip(g_sq, g0):
+        for ta, tb in zip(a, b):
+            np.testing.assert_array_equal(ta.numpy(), tb.numpy())
 
 
 def test_fit_100_steps_with_fp64_error_per_step():
