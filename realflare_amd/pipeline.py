@@ -41,6 +41,7 @@ class FramePipeline:
                  pair_capacity: int | None = None, device=None) -> None:
         self.device = device or torch.device('cuda', torch.cuda.current_device())
         dev = self.device
+        self._side = None  # second stream of render(), made on first use
         self.project = project
         lens = project.flare.lens
         render = project.render
@@ -167,9 +168,38 @@ class FramePipeline:
                                             s())),
         ]
 
+    # stages that do not depend on the trace: the ghost texture (needed by the
+    # rasterizer) and the starburst chain (needed by the composition)
+    _SIDE_GHOST = ('ghost_aperture', 'ghost')
+    _SIDE_STAR = ('starburst_aperture', 'fourier_spectrum', 'starburst')
+
     def render(self) -> torch.Tensor:
-        for _, fn in self.stage_fns():
-            fn()
+        """One frame.  The ghost texture and the starburst run on a second
+        stream (default priority) beside the trace and the binner; the
+        rasterizer waits for the texture and the composition for the
+        starburst.  The side stream first waits for the previous work of the
+        main stream, so consecutive frames never overlap on shared buffers."""
+        fns = dict(self.stage_fns())
+        main = torch.cuda.current_stream()
+        if self._side is None:
+            self._side = torch.cuda.Stream()
+        side = self._side
+        side.wait_stream(main)
+        with torch.cuda.stream(side):
+            for name in self._SIDE_GHOST:
+                fns[name]()
+            ghost_done = torch.cuda.Event()
+            ghost_done.record(side)
+            for name in self._SIDE_STAR:
+                fns[name]()
+            star_done = torch.cuda.Event()
+            star_done.record(side)
+        for name in ('raytrace', 'prim_shader', 'vertex_shader', 'binner'):
+            fns[name]()
+        main.wait_event(ghost_done)
+        fns['rasterizer']()
+        main.wait_event(star_done)
+        fns['compose']()
         return self.out
 
     def render_timed(self, events: dict | None = None) -> torch.Tensor:
