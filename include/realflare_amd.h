@@ -249,7 +249,7 @@ typedef struct rf_raster_params {
     float screen_transform;       /* width / half sensor diagonal (py:158-167) */
     float intensity;              /* light.intensity * 1e3 (py:415) */
     float ghost_scale;            /* 1 - lens.fstop / 32 (py:393) */
-    int32_t ghost_width, ghost_height;
+    int32_t ghost_width, ghost_height; /* at most 2^28 texels, each side below 2^24 - 4 */
     int32_t spectrum_count;       /* texels of the light spectrum, RGBA each */
     int32_t accumulate;           /* 0: write the image (the reference); 1: add into it */
     int32_t super_block;          /* 8x8-pixel tiles per side of the groups the rasterizer

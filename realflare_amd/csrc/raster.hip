@@ -843,34 +843,52 @@ __device__ __forceinline__ void barycentric_fast(int px, int py, const int* x, c
 }
 
 // Bilinear lookup of the one-channel ghost texture with border 0 (the OpenCL
-// 1.2 formula of sample_bilinear), for shading: no branches.  The texel
-// coordinate is clamped to [-2, size + 1] before the conversion, which keeps
-// every texel outside the texture outside it (so it still reads 0) and maps
-// NaN to a border texel (the weights are then NaN, as in the source).  The
-// texture is read through a buffer resource sized to it: a texel outside the
-// texture gets an offset past the end, which the hardware range check
-// answers with 0.
+// 1.2 formula of sample_bilinear), for shading: no branches.  The lookup reads
+// a copy of the texture framed by GHOST_PAD zero texels on every side (row
+// pitch w + 2 GHOST_PAD texels, written by k_pad_ghost).  The texel
+// coordinate is clamped to [-2, size] before the conversion: a clamped
+// coordinate reads only border texels, as the unclamped one would (both
+// texels of an axis lie outside the texture once the coordinate is at or past
+// size, or at or below -2), every texel it reads lies in the frame, so no
+// texel needs a range test, and NaN becomes a border texel (the weights are
+// then NaN, as in the source).  The two texels of a row are one 8-byte load;
+// the second row is the same offset plus the pitch, passed as the load's
+// scalar offset.
 static constexpr int RF_BUFFER_WORD3 = 0x00020000;  // raw 32-bit buffer, gfx9 descriptor
+static constexpr int GHOST_PAD = 2;
 
-__device__ __forceinline__ float ghost_texel(__amdgpu_buffer_rsrc_t tex, int w, int h, int i, int j) {
-    const bool in = (uint32_t)i < (uint32_t)w && (uint32_t)j < (uint32_t)h;
-    const uint32_t off = in ? (uint32_t)(j * w + i) * 4u : 0x80000000u;
-    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(tex, off, 0, 0));
-}
+typedef uint32_t rf_u32x2 __attribute__((ext_vector_type(2)));
 
 // (u, v) in texel units: s * w - 0.5, t * h - 0.5.  Blended as two lerps
 // along u and one along v (the source's four-product form, reassociated).
-__device__ __forceinline__ float ghost_sample(__amdgpu_buffer_rsrc_t tex, int w, int h, float u,
-                                              float v) {
+// pitch = w + 2 GHOST_PAD; w + 2 GHOST_PAD and h + 2 GHOST_PAD are below 2^24,
+// so the 24-bit multiply is exact.
+__device__ __forceinline__ float ghost_sample(__amdgpu_buffer_rsrc_t tex, int w, int h, int pitch,
+                                              float u, float v) {
 #pragma clang fp contract(fast)
     const float fu = floorf(u), fv = floorf(v);
-    const int i0 = (int)fminf(fmaxf(fu, -2.0f), (float)w + 1.0f);
-    const int j0 = (int)fminf(fmaxf(fv, -2.0f), (float)h + 1.0f);
+    const int i0 = (int)fminf(fmaxf(fu, -2.0f), (float)w);
+    const int j0 = (int)fminf(fmaxf(fv, -2.0f), (float)h);
     const float a = u - fu, b = v - fv;
-    const float t00 = ghost_texel(tex, w, h, i0, j0), t10 = ghost_texel(tex, w, h, i0 + 1, j0);
-    const float t01 = ghost_texel(tex, w, h, i0, j0 + 1), t11 = ghost_texel(tex, w, h, i0 + 1, j0 + 1);
+    const uint32_t off =
+        ((uint32_t)__umul24((uint32_t)(j0 + GHOST_PAD), (uint32_t)pitch) + (uint32_t)(i0 + GHOST_PAD)) * 4u;
+    const rf_u32x2 r0 = __builtin_amdgcn_raw_buffer_load_b64(tex, off, 0, 0);
+    const rf_u32x2 r1 = __builtin_amdgcn_raw_buffer_load_b64(tex, off, pitch * 4, 0);
+    const float t00 = __uint_as_float(r0.x), t10 = __uint_as_float(r0.y);
+    const float t01 = __uint_as_float(r1.x), t11 = __uint_as_float(r1.y);
     const float top = t00 + a * (t10 - t00), bottom = t01 + a * (t11 - t01);
     return top + b * (bottom - top);
+}
+
+// The ghost texture copied into the centre of a frame of GHOST_PAD zero texels.
+__global__ __launch_bounds__(256) void k_pad_ghost(const float* __restrict__ ghost, int w, int h,
+                                                   float* __restrict__ padded) {
+    const int pw = w + 2 * GHOST_PAD, ph = h + 2 * GHOST_PAD;
+    const long n = (long)pw * ph;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        const int x = (int)(i % pw) - GHOST_PAD, y = (int)(i / pw) - GHOST_PAD;
+        padded[i] = (x >= 0 && x < w && y >= 0 && y < h) ? ghost[(long)y * w + x] : 0.0f;
+    }
 }
 
 // Workgroups are dealt to the XCDs round-robin (workgroup i runs on XCD
@@ -958,7 +976,8 @@ __device__ __forceinline__ float shade_fragment(int pcx, int pcy, const Setup& r
     barycentric_fast(pcx, pcy, rec.vx, rec.vy, w);
     const float uu = w[0] * at.u[0] + w[1] * at.u[1] + w[2] * at.u[2] + w[3] * at.u[3];
     const float vv = w[0] * at.v[0] + w[1] * at.v[1] + w[2] * at.v[2] + w[3] * at.v[3];
-    const float gi = ghost_sample(ghost, G.gw, G.gh, uu * gm.ku + gm.cu, vv * gm.kv + gm.cv);
+    const float gi = ghost_sample(ghost, G.gw, G.gh, G.gw + 2 * GHOST_PAD, uu * gm.ku + gm.cu,
+                                  vv * gm.kv + gm.cv);
     const float rel = w[0] * at.rrel[0] + w[1] * at.rrel[1] + w[2] * at.rrel[2] + w[3] * at.rrel[3];
     // smoothstep(1, 0.95, rrel)
     constexpr float inv_edge = 1.0f / (0.95f - 1.0f);
@@ -990,7 +1009,7 @@ static constexpr uint32_t QCAP = 128;
 // exactly the terms of the per-pixel loop in the same order.  Shading thus runs
 // on full waves even when a quad covers few pixels of the tile.
 __global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ bounds,
-                                               const float* __restrict__ ghost,
+                                               const float* __restrict__ ghost_pad,
                                                const Setup* __restrict__ setup,
                                                const Edges* __restrict__ edges,
                                                const Attr* __restrict__ attr,
@@ -1018,7 +1037,8 @@ __global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ bounds
     // queue state (wave-uniform); slot s (the s-th queued pair) in s_slot[s]
     uint32_t qlen = 0, nslot = 0;
     const __amdgpu_buffer_rsrc_t ghost_rsrc = __builtin_amdgcn_make_buffer_rsrc(
-        (void*)ghost, (short)0, G.gw * G.gh * (int)sizeof(float), RF_BUFFER_WORD3);
+        (void*)ghost_pad, (short)0,
+        (G.gw + 2 * GHOST_PAD) * (G.gh + 2 * GHOST_PAD) * (int)sizeof(float), RF_BUFFER_WORD3);
 
     auto flush = [&]() {
         __syncthreads();  // queue entries written by other lanes are visible
@@ -1133,6 +1153,9 @@ static int make_geom(const rf_raster_params* p, Geom* G) {
     if ((long)p->ghost_width * p->ghost_height > (1L << 28))
         return set_error(RF_ERR_INVALID, "raster: ghost texture of %dx%d texels exceeds 2^28",
                          p->ghost_width, p->ghost_height);
+    if (p->ghost_width > (1 << 24) - 2 * GHOST_PAD - 1 || p->ghost_height > (1 << 24) - 2 * GHOST_PAD - 1)
+        return set_error(RF_ERR_INVALID, "raster: ghost texture side of %dx%d reaches 2^24",
+                         p->ghost_width, p->ghost_height);
     G->P = p->path_count;
     G->W = p->wavelength_count;
     G->g = p->grid_count;
@@ -1187,7 +1210,7 @@ static inline unsigned blocks_for(long n, int threads) { return (unsigned)((n + 
 
 struct Layout {
     size_t header, table, setup, attr, edges, heavy, order_keys, order_vals, counts, offsets,
-        scan_ws, keys, vals, radix_ws, tstart, tend, total;
+        scan_ws, keys, vals, radix_ws, tstart, tend, ghost_pad, total;
 };
 
 static Layout layout(const Geom& G, uint64_t cap) {
@@ -1216,6 +1239,7 @@ static Layout layout(const Geom& G, uint64_t cap) {
     const size_t ntiles = (size_t)G.ntx * G.nty * G.frames;
     L.tstart = take(ntiles * sizeof(uint32_t));
     L.tend = take(ntiles * sizeof(uint32_t));
+    L.ghost_pad = take((size_t)(G.gw + 2 * GHOST_PAD) * (G.gh + 2 * GHOST_PAD) * sizeof(float));
     L.total = o;
     return L;
 }
@@ -1231,6 +1255,7 @@ struct RasterPtrs {
     uint32_t *counts, *keys, *vals, *tstart, *tend;
     uint64_t* offsets;
     void *scan_ws, *radix_ws;
+    float* ghost_pad;
 };
 
 static int raster_ptrs(const Geom& G, uint64_t cap, void* ws, size_t ws_bytes, RasterPtrs* r,
@@ -1256,6 +1281,7 @@ static int raster_ptrs(const Geom& G, uint64_t cap, void* ws, size_t ws_bytes, R
     r->radix_ws = w + L.radix_ws;
     r->tstart = (uint32_t*)(w + L.tstart);
     r->tend = (uint32_t*)(w + L.tend);
+    r->ghost_pad = (float*)(w + L.ghost_pad);
     return RF_OK;
 }
 
@@ -1308,7 +1334,10 @@ static int draw_impl(const float4* vtx, const float4* bounds, const float* ghost
     const long ntiles = (long)G.ntx * G.nty;
     (void)vtx;
     (void)ntiles;
-    hipLaunchKernelGGL(k_raster, dim3((unsigned)raster_grid(G)), dim3(64), 0, s, bounds, ghost,
+    const long texels = (long)(G.gw + 2 * GHOST_PAD) * (G.gh + 2 * GHOST_PAD);
+    hipLaunchKernelGGL(k_pad_ghost, dim3((unsigned)min(blocks_for(texels, 256), 4096u)), dim3(256), 0,
+                       s, ghost, G.gw, G.gh, r.ghost_pad);
+    hipLaunchKernelGGL(k_raster, dim3((unsigned)raster_grid(G)), dim3(64), 0, s, bounds, r.ghost_pad,
                        r.setup, r.edges, r.attr, r.vals, r.tstart, r.tend, r.order_vals, r.h, G,
                        image);
     RF_LAUNCH_CHECK("rasterizer");
