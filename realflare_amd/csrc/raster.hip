@@ -785,6 +785,21 @@ __device__ __forceinline__ int edge_row(const Edges& er, int k, int px, int py) 
     return (int)((uint32_t)__mul24(er.a[k], py) + (uint32_t)__mul24(er.nb[k], px) + (uint32_t)er.c[k]);
 }
 
+// cover_small for every lane of the wave at once, as a lane mask: one
+// comparison per edge gives the edge's sign bits of all 64 samples, and the
+// front / back tests combine the five masks with wave-uniform (scalar) bit
+// operations.  Every lane of the wave must be active.
+__device__ __forceinline__ uint64_t cover_small_mask(int px, int py, const Edges& er) {
+    const uint64_t w01 = __ballot(edge_row(er, 0, px, py) >= 0);
+    const uint64_t w12 = __ballot(edge_row(er, 1, px, py) >= 0);
+    const uint64_t w23 = __ballot(edge_row(er, 2, px, py) >= 0);
+    const uint64_t w30 = __ballot(edge_row(er, 3, px, py) >= 0);
+    const uint64_t w20 = __ballot(edge_row(er, 4, px, py) >= 0);
+    const uint64_t front = (w20 & w01 & w12 & (w23 | w30)) | (~w20 & w23 & w30 & (w01 | w12));
+    const uint64_t back = (~w20 & ~w01 & ~w12 & (~w23 | ~w30)) | (w20 & ~w23 & ~w30 & (~w01 | ~w12));
+    return front | back;
+}
+
 __device__ __forceinline__ bool cover_small(int px, int py, const Edges& er) {
     const bool w01 = edge_row(er, 0, px, py) >= 0;
     const bool w12 = edge_row(er, 1, px, py) >= 0;
@@ -1077,41 +1092,52 @@ __global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ bounds
     };
 
     const uint32_t start = tstart[tile], end = tend[tile];
+    const uint64_t inside_mask = __ballot(inside);
+    // one sample per pixel and bins made of whole tiles: the coverage of a
+    // FLAG_SMALL quad step is computed as a lane mask (cover_small_mask)
+    const bool mask_path = G.AA == 1 && G.bin_exact;
     for (uint32_t k = start; k < end; k++) {
         const uint32_t e = vals[k];
         const Edges er = edges[e];
         int hits = 0;
-        bool ok = inside;
-        if (ok && !G.bin_exact) {
-            const float4 b = bounds[setup[e].prim];
-            const int sbx0 = wadd(rf_to_int(b.x * G.st), G.width / 2);
-            const int sby0 = wadd(rf_to_int(b.y * G.st), G.height / 2);
-            const int sbx1 = wadd(rf_to_int(b.z * G.st), G.width / 2);
-            const int sby1 = wadd(rf_to_int(b.w * G.st), G.height / 2);
-            const int bx0 = bxp * G.bin, by0 = byp * G.bin;
-            ok = sbx0 <= bx0 + G.bin && sbx1 >= bx0 && sby0 <= by0 + G.bin && sby1 >= by0;
-        }
-        if (ok) {
-            if (er.flags & FLAG_SMALL) {
-                if (G.AA == 1) {
-                    hits = cover_small(px, py, er) ? 1 : 0;
+        uint64_t m;
+        if (mask_path && (er.flags & FLAG_SMALL)) {
+            m = cover_small_mask(px, py, er) & inside_mask;
+            hits = 1;  // read by the covered lanes only
+        } else {
+            bool ok = inside;
+            if (ok && !G.bin_exact) {
+                const float4 b = bounds[setup[e].prim];
+                const int sbx0 = wadd(rf_to_int(b.x * G.st), G.width / 2);
+                const int sby0 = wadd(rf_to_int(b.y * G.st), G.height / 2);
+                const int sbx1 = wadd(rf_to_int(b.z * G.st), G.width / 2);
+                const int sby1 = wadd(rf_to_int(b.w * G.st), G.height / 2);
+                const int bx0 = bxp * G.bin, by0 = byp * G.bin;
+                ok = sbx0 <= bx0 + G.bin && sbx1 >= bx0 && sby0 <= by0 + G.bin && sby1 >= by0;
+            }
+            if (ok) {
+                if (er.flags & FLAG_SMALL) {
+                    if (G.AA == 1) {
+                        hits = cover_small(px, py, er) ? 1 : 0;
+                    } else {
+                        for (int s = 0; s < G.AA; s++)
+                            hits += cover_small(px + s, py + sub_offset(G.AA - 1 + s), er) ? 1 : 0;
+                    }
                 } else {
-                    for (int s = 0; s < G.AA; s++)
-                        hits += cover_small(px + s, py + sub_offset(G.AA - 1 + s), er) ? 1 : 0;
-                }
-            } else {
-                const Setup rec = setup[e];
-                for (int s = 0; s < G.AA; s++) {
-                    const int sx = px + s, sy = py + sub_offset(G.AA - 1 + s);
-                    if (intersect_quad(sx, sy, rec.vx, rec.vy, rec.flags)) hits++;
+                    const Setup rec = setup[e];
+                    for (int s = 0; s < G.AA; s++) {
+                        const int sx = px + s, sy = py + sub_offset(G.AA - 1 + s);
+                        if (intersect_quad(sx, sy, rec.vx, rec.vy, rec.flags)) hits++;
+                    }
                 }
             }
+            m = __ballot(hits > 0);
         }
-        const uint64_t m = __ballot(hits > 0);
         if (m == 0) continue;
         const uint32_t n = (uint32_t)__popcll(m);
         if (qlen + n > QCAP) flush();
-        if (hits > 0) s_q[qlen + lanes_below(m)] = (uint32_t)lane | (uint32_t)hits << 6 | nslot << 10;
+        if ((m >> lane) & 1)
+            s_q[qlen + lanes_below(m)] = (uint32_t)lane | (uint32_t)hits << 6 | nslot << 10;
         if (lane == 0) s_slot[nslot] = make_uint4((uint32_t)m, (uint32_t)(m >> 32), qlen, e);
         qlen += n;
         nslot++;
