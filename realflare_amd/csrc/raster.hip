@@ -48,6 +48,11 @@ struct Geom {
     long NQ;                   // quads of this pass (plain: P*Q)
     long NR;                   // vertices of this pass (plain: N)
     float min_area_factor;     // batch: lens.min_area (scaled per segment)
+    // Ghost texture lookup, set by make_geom so the rasterizer reads them as
+    // scalars: texel coordinate = uv * k + c, folding the (uv / ghost_scale
+    // + 1) / 2 of rasterizing.cl and the texel-centre offset; gwf, ghf = the
+    // texture size as floats (the upper clamp of the lookup).
+    float gku, gcu, gkv, gcv, gwf, ghf;
 };
 
 // Where a ghost's data lives: first ray (planes [W][R], vertices [R][W]),
@@ -874,16 +879,16 @@ static constexpr int GHOST_PAD = 2;
 
 typedef uint32_t rf_u32x2 __attribute__((ext_vector_type(2)));
 
-// (u, v) in texel units: s * w - 0.5, t * h - 0.5.  Blended as two lerps
-// along u and one along v (the source's four-product form, reassociated).
-// pitch = w + 2 GHOST_PAD; w + 2 GHOST_PAD and h + 2 GHOST_PAD are below 2^24,
+// (u, v) in texel units: s * w - 0.5, t * h - 0.5; w, h = the texture size.
+// Blended as two lerps along u and one along v (the source's four-product
+// form, reassociated).  pitch = w + 2 GHOST_PAD; w + 2 GHOST_PAD and h + 2 GHOST_PAD are below 2^24,
 // so the 24-bit multiply is exact.
-__device__ __forceinline__ float ghost_sample(__amdgpu_buffer_rsrc_t tex, int w, int h, int pitch,
+__device__ __forceinline__ float ghost_sample(__amdgpu_buffer_rsrc_t tex, float w, float h, int pitch,
                                               float u, float v) {
 #pragma clang fp contract(fast)
     const float fu = floorf(u), fv = floorf(v);
-    const int i0 = (int)fminf(fmaxf(fu, -2.0f), (float)w);
-    const int j0 = (int)fminf(fmaxf(fv, -2.0f), (float)h);
+    const int i0 = (int)fminf(fmaxf(fu, -2.0f), w);
+    const int j0 = (int)fminf(fmaxf(fv, -2.0f), h);
     const float a = u - fu, b = v - fv;
     const uint32_t off =
         ((uint32_t)__umul24((uint32_t)(j0 + GHOST_PAD), (uint32_t)pitch) + (uint32_t)(i0 + GHOST_PAD)) * 4u;
@@ -971,28 +976,16 @@ __global__ __launch_bounds__(256) void k_block_cost(const uint32_t* __restrict__
 
 // fragment_shader (rasterizing.cl:471-506) for one covered pixel of one quad
 // step, scaled by its covered AA samples; fused multiply-adds allowed.
-// Ghost texture lookup constants: texel coordinate = uv * k + c, folding the
-// (uv / ghost_scale + 1) / 2 of rasterizing.cl and the texel-centre offset.
-struct GhostMap {
-    float ku, cu, kv, cv;
-};
-
-__device__ __forceinline__ GhostMap ghost_map(const Geom& G) {
-    const float half_inv = 0.5f / G.ghost_scale;
-    return {half_inv * (float)G.gw, 0.5f * (float)G.gw - 0.5f, half_inv * (float)G.gh,
-            0.5f * (float)G.gh - 0.5f};
-}
-
 __device__ __forceinline__ float shade_fragment(int pcx, int pcy, const Setup& rec, const Attr& at,
                                                 int hits, __amdgpu_buffer_rsrc_t ghost,
-                                                const Geom& G, const GhostMap& gm) {
+                                                const Geom& G) {
 #pragma clang fp contract(fast)
     float w[4];
     barycentric_fast(pcx, pcy, rec.vx, rec.vy, w);
     const float uu = w[0] * at.u[0] + w[1] * at.u[1] + w[2] * at.u[2] + w[3] * at.u[3];
     const float vv = w[0] * at.v[0] + w[1] * at.v[1] + w[2] * at.v[2] + w[3] * at.v[3];
-    const float gi = ghost_sample(ghost, G.gw, G.gh, G.gw + 2 * GHOST_PAD, uu * gm.ku + gm.cu,
-                                  vv * gm.kv + gm.cv);
+    const float gi = ghost_sample(ghost, G.gwf, G.ghf, G.gw + 2 * GHOST_PAD, uu * G.gku + G.gcu,
+                                  vv * G.gkv + G.gcv);
     const float rel = w[0] * at.rrel[0] + w[1] * at.rrel[1] + w[2] * at.rrel[2] + w[3] * at.rrel[3];
     // smoothstep(1, 0.95, rrel)
     constexpr float inv_edge = 1.0f / (0.95f - 1.0f);
@@ -1023,6 +1016,10 @@ static constexpr uint32_t QCAP = 128;
 // to their pixel lanes slot by slot, i.e. in pair order, so each pixel adds
 // exactly the terms of the per-pixel loop in the same order.  Shading thus runs
 // on full waves even when a quad covers few pixels of the tile.
+// kMask: one sample per pixel and bins made of whole tiles (G.AA == 1,
+// G.bin_exact), the common configuration; its instantiation carries neither
+// the bin re-test nor the multi-sample loops, and so holds fewer registers.
+template <bool kMask>
 __global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ bounds,
                                                const float* __restrict__ ghost_pad,
                                                const Setup* __restrict__ setup,
@@ -1045,9 +1042,8 @@ __global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ bounds
     const int lane = threadIdx.x;
     const int x = tx * TILE + (lane & (TILE - 1)), y = ty * TILE + (lane / TILE);
     const bool inside = x < G.width && y < G.height;
-    const int px = x * G.AA, py = y * G.AA;
-    const int bxp = x / G.bin, byp = y / G.bin;
-    const GhostMap gmap = ghost_map(G);
+    const int AA = kMask ? 1 : G.AA;
+    const int px = x * AA, py = y * AA;
     float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f;
     // queue state (wave-uniform); slot s (the s-th queued pair) in s_slot[s]
     uint32_t qlen = 0, nslot = 0;
@@ -1067,8 +1063,8 @@ __global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ bounds
                 const Setup rec = setup[e];
                 const Attr at = attr[e];
                 const int fx = tx * TILE + (p & (TILE - 1)), fy = ty * TILE + p / TILE;
-                const float frag = shade_fragment(fx * G.AA + G.AA / 2, fy * G.AA + G.AA / 2, rec,
-                                                  at, hits, ghost_rsrc, G, gmap);
+                const float frag = shade_fragment(fx * AA + AA / 2, fy * AA + AA / 2, rec,
+                                                  at, hits, ghost_rsrc, G);
                 c0 = at.x * frag;
                 c1 = at.y * frag;
                 c2 = at.z * frag;
@@ -1093,40 +1089,39 @@ __global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ bounds
 
     const uint32_t start = tstart[tile], end = tend[tile];
     const uint64_t inside_mask = __ballot(inside);
-    // one sample per pixel and bins made of whole tiles: the coverage of a
-    // FLAG_SMALL quad step is computed as a lane mask (cover_small_mask)
-    const bool mask_path = G.AA == 1 && G.bin_exact;
     for (uint32_t k = start; k < end; k++) {
         const uint32_t e = vals[k];
         const Edges er = edges[e];
         int hits = 0;
         uint64_t m;
-        if (mask_path && (er.flags & FLAG_SMALL)) {
+        // one sample per pixel and bins made of whole tiles: the coverage of a
+        // FLAG_SMALL quad step is computed as a lane mask (cover_small_mask)
+        if (kMask && (er.flags & FLAG_SMALL)) {
             m = cover_small_mask(px, py, er) & inside_mask;
             hits = 1;  // read by the covered lanes only
         } else {
             bool ok = inside;
-            if (ok && !G.bin_exact) {
+            if (!kMask && ok && !G.bin_exact) {
                 const float4 b = bounds[setup[e].prim];
                 const int sbx0 = wadd(rf_to_int(b.x * G.st), G.width / 2);
                 const int sby0 = wadd(rf_to_int(b.y * G.st), G.height / 2);
                 const int sbx1 = wadd(rf_to_int(b.z * G.st), G.width / 2);
                 const int sby1 = wadd(rf_to_int(b.w * G.st), G.height / 2);
-                const int bx0 = bxp * G.bin, by0 = byp * G.bin;
+                const int bx0 = x / G.bin * G.bin, by0 = y / G.bin * G.bin;
                 ok = sbx0 <= bx0 + G.bin && sbx1 >= bx0 && sby0 <= by0 + G.bin && sby1 >= by0;
             }
             if (ok) {
                 if (er.flags & FLAG_SMALL) {
-                    if (G.AA == 1) {
+                    if (AA == 1) {
                         hits = cover_small(px, py, er) ? 1 : 0;
                     } else {
-                        for (int s = 0; s < G.AA; s++)
-                            hits += cover_small(px + s, py + sub_offset(G.AA - 1 + s), er) ? 1 : 0;
+                        for (int s = 0; s < AA; s++)
+                            hits += cover_small(px + s, py + sub_offset(AA - 1 + s), er) ? 1 : 0;
                     }
                 } else {
                     const Setup rec = setup[e];
-                    for (int s = 0; s < G.AA; s++) {
-                        const int sx = px + s, sy = py + sub_offset(G.AA - 1 + s);
+                    for (int s = 0; s < AA; s++) {
+                        const int sx = px + s, sy = py + sub_offset(AA - 1 + s);
                         if (intersect_quad(sx, sy, rec.vx, rec.vy, rec.flags)) hits++;
                     }
                 }
@@ -1146,7 +1141,7 @@ __global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ bounds
     if (!inside) return;
     float4 out = make_float4(0, 0, 0, 0);
     if (r0 > 0 || r1 > 0 || r2 > 0) {
-        const float f = G.intensity / (float)(G.W * G.AA * G.Wsub);
+        const float f = G.intensity / (float)(G.W * AA * G.Wsub);
         float o[4];
         xyz_to_ap1(r0 * f, r1 * f, r2 * f, o);
         out = make_float4(o[0], o[1], o[2], o[3]);
@@ -1225,6 +1220,13 @@ static int make_geom(const rf_raster_params* p, Geom* G) {
     G->NQ = (long)G->P * G->Q;
     G->NR = G->N;
     G->min_area_factor = 0.0f;
+    const float half_inv = 0.5f / G->ghost_scale;
+    G->gku = half_inv * (float)G->gw;
+    G->gcu = 0.5f * (float)G->gw - 0.5f;
+    G->gkv = half_inv * (float)G->gh;
+    G->gcv = 0.5f * (float)G->gh - 0.5f;
+    G->gwf = (float)G->gw;
+    G->ghf = (float)G->gh;
     if (G->W > 1 && G->nspec < 1)
         return set_error(RF_ERR_INVALID, "raster: spectrum_count must be >= 1");
     if (G->NE > 0xffffffffL)
@@ -1363,9 +1365,14 @@ static int draw_impl(const float4* vtx, const float4* bounds, const float* ghost
     const long texels = (long)(G.gw + 2 * GHOST_PAD) * (G.gh + 2 * GHOST_PAD);
     hipLaunchKernelGGL(k_pad_ghost, dim3((unsigned)min(blocks_for(texels, 256), 4096u)), dim3(256), 0,
                        s, ghost, G.gw, G.gh, r.ghost_pad);
-    hipLaunchKernelGGL(k_raster, dim3((unsigned)raster_grid(G)), dim3(64), 0, s, bounds, r.ghost_pad,
-                       r.setup, r.edges, r.attr, r.vals, r.tstart, r.tend, r.order_vals, r.h, G,
-                       image);
+    if (G.AA == 1 && G.bin_exact)
+        hipLaunchKernelGGL(k_raster<true>, dim3((unsigned)raster_grid(G)), dim3(64), 0, s, bounds,
+                           r.ghost_pad, r.setup, r.edges, r.attr, r.vals, r.tstart, r.tend,
+                           r.order_vals, r.h, G, image);
+    else
+        hipLaunchKernelGGL(k_raster<false>, dim3((unsigned)raster_grid(G)), dim3(64), 0, s, bounds,
+                           r.ghost_pad, r.setup, r.edges, r.attr, r.vals, r.tstart, r.tend,
+                           r.order_vals, r.h, G, image);
     RF_LAUNCH_CHECK("rasterizer");
     return RF_OK;
 }
