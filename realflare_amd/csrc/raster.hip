@@ -156,6 +156,7 @@ struct alignas(16) Edges {
     int a[5], nb[5], c[5];
     int flags;  // as Setup.flags
 };
+static_assert(sizeof(Edges) == 64, "one edge-row record per 64-byte line");
 
 // Shading inputs of one (quad, step): the four corners' attributes blended
 // between the step's two wavelengths (mix_vertex, rasterizing.cl:509-522)
@@ -1008,6 +1009,10 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
 // (256 entries, 9 KB) would lower it.
 static constexpr uint32_t QCAP = 128;
 
+// Pairs between a pair whose edge rows have arrived and the pair whose rows
+// are then pulled into the scalar cache (k_raster's walk).
+static constexpr uint32_t TOUCH = 2;
+
 // rasterizer (rasterizing.cl:524-663): one wave per 8x8 tile, lane = pixel.
 // The tile's (quad step) pairs are walked in binning order.  A pair's coverage
 // is tested by every pixel lane; the covered pixels are appended to a QCAP-entry
@@ -1089,9 +1094,29 @@ __global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ bounds
 
     const uint32_t start = tstart[tile], end = tend[tile];
     const uint64_t inside_mask = __ballot(inside);
+    // The pair ids are read 64 at a time by one vector load (lane l holds the
+    // id of pair start + 64 b + l) and taken per pair with readlane, so no
+    // scalar load of an id precedes the rows of a pair.  The scalar load of a
+    // pair's edge rows is then the walk's only latency: once the rows of pair
+    // k have arrived, one dword of the rows of pair k + TOUCH is loaded, which
+    // brings that pair's 64-byte record into the scalar cache while pairs
+    // k .. k + TOUCH - 1 are processed.  Its address depends on the rows just
+    // loaded: scalar loads complete out of order, so a wait covers every one
+    // in flight, and a touch issued before the rows' wait would be waited for
+    // with them.  The touched value is consumed by an empty asm statement
+    // after the wait of a later pair, so the load is kept and costs no wait.
+    uint32_t ids = 0, touched = 0;
     for (uint32_t k = start; k < end; k++) {
-        const uint32_t e = vals[k];
+        const uint32_t j = (k - start) & 63;
+        if (j == 0) ids = k + (uint32_t)lane < end ? vals[k + lane] : 0u;
+        const uint32_t e = (uint32_t)__builtin_amdgcn_readlane((int)ids, (int)j);
         const Edges er = edges[e];
+        if (j < 64 - TOUCH && k + TOUCH < end) {
+            const uint32_t en = (uint32_t)__builtin_amdgcn_readlane((int)ids, (int)(j + TOUCH)) ^
+                                ((uint32_t)er.flags >> 31);  // flags < 64: en is the id itself
+            asm volatile("" ::"s"(touched));
+            touched = reinterpret_cast<const uint32_t*>(edges)[(size_t)en * (sizeof(Edges) / 4)];
+        }
         int hits = 0;
         uint64_t m;
         // one sample per pixel and bins made of whole tiles: the coverage of a
