@@ -117,7 +117,7 @@ def cpu_baseline(pipe, inp_name: str = 'full') -> dict:
     vtx = oracle.vertex_shader(rays, intens, g, prm.screen_transform, pipe.resolution)
     t_pv = time.perf_counter() - t0
     w, h = pipe.resolution
-    rows = [int((i + 0.5) * h / 6) for i in range(6)]
+    rows = [int((i + 0.5) * h / 24) for i in range(24)]
     t0 = time.perf_counter()
     for y in rows:
         oracle.rasterize(vtx, bounds, prm.wavelength_sub_count, g, prm.anti_aliasing,

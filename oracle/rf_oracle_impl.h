@@ -708,7 +708,9 @@ int FN(rfo_rasterize)(const VERTEX *vertexes, const REAL *bounds, int P, int W, 
         lists[b] = l;
         counts[b] = n;
     }
-#pragma omp parallel for schedule(dynamic, 1)
+    /* (row, pixel) pairs are shared among the threads, so a band of a few
+     * rows uses every thread too */
+#pragma omp parallel for collapse(2) schedule(dynamic, 64)
     for (int y = y0; y < y1; y++) {
         for (int x = 0; x < width; x++) {
             long b = (long)(y / bin_size) * bdx + x / bin_size;
