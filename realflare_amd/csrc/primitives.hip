@@ -127,24 +127,42 @@ static constexpr int RS_THREADS = 256;
 static constexpr int RS_ITEMS = 16;
 static constexpr int RS_CHUNK = RS_THREADS * RS_ITEMS;  // 4096
 
+// Digit counts of one 4096-item chunk.  Keys are read four at a time (the
+// key array and every chunk start are 16-byte aligned) and counted in one LDS
+// histogram per wave: neighbouring pairs often share a digit, and one shared
+// histogram would serialize their atomics.
 __global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const uint32_t* __restrict__ keys,
                                                             const uint64_t* __restrict__ n_ptr,
                                                             int shift, int nblocks,
                                                             uint32_t* __restrict__ hist) {
-    __shared__ uint32_t s_hist[256];
-    s_hist[threadIdx.x] = 0;
+    __shared__ uint32_t s_hist[RS_THREADS / 64][256];
+    const int wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int w = 0; w < RS_THREADS / 64; w++) s_hist[w][threadIdx.x] = 0;
     __syncthreads();
     const uint64_t n = *n_ptr;
     const uint64_t base = (uint64_t)blockIdx.x * RS_CHUNK;
     if (base < n) {
-        for (int i = 0; i < RS_ITEMS; i++) {
-            uint64_t idx = base + (uint64_t)i * RS_THREADS + threadIdx.x;
-            if (idx < n) atomicAdd(&s_hist[(keys[idx] >> shift) & 255u], 1u);
+        for (int i = 0; i < RS_ITEMS / 4; i++) {
+            const uint64_t idx = base + 4 * ((uint64_t)i * RS_THREADS + threadIdx.x);
+            if (idx + 3 < n) {
+                const uint4 k = *reinterpret_cast<const uint4*>(keys + idx);
+                atomicAdd(&s_hist[wave][(k.x >> shift) & 255u], 1u);
+                atomicAdd(&s_hist[wave][(k.y >> shift) & 255u], 1u);
+                atomicAdd(&s_hist[wave][(k.z >> shift) & 255u], 1u);
+                atomicAdd(&s_hist[wave][(k.w >> shift) & 255u], 1u);
+            } else {
+                for (uint64_t j = idx; j < n && j < idx + 4; j++)
+                    atomicAdd(&s_hist[wave][(keys[j] >> shift) & 255u], 1u);
+            }
         }
     }
     __syncthreads();
+    uint32_t c = 0;
+#pragma unroll
+    for (int w = 0; w < RS_THREADS / 64; w++) c += s_hist[w][threadIdx.x];
     // digit-major: all blocks of digit 0, then digit 1, ...
-    hist[(uint64_t)threadIdx.x * nblocks + blockIdx.x] = s_hist[threadIdx.x];
+    hist[(uint64_t)threadIdx.x * nblocks + blockIdx.x] = c;
 }
 
 // Scatter of one 4096-item chunk: items are first ranked (stable: by digit,
@@ -161,6 +179,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
     __shared__ uint32_t s_run[256];  // items of each digit placed so far
     __shared__ uint32_t s_loc[256];  // first LDS slot of each digit
     __shared__ uint32_t s_wcnt[RS_THREADS / 64][256];
+    __shared__ uint64_t s_doff[256];  // output offset of each digit's run of this chunk
     const uint64_t n = *n_ptr;
     const uint64_t base = (uint64_t)blockIdx.x * RS_CHUNK;
     if (base >= n) return;
@@ -168,6 +187,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
     const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
     // exclusive scan of the chunk's digit counts (thread t owns digit t)
     const uint32_t cnt = hist[(uint64_t)threadIdx.x * nblocks + blockIdx.x];
+    s_doff[threadIdx.x] = digit_offs[(uint64_t)threadIdx.x * nblocks + blockIdx.x];
     s_loc[threadIdx.x] = cnt;
     s_run[threadIdx.x] = 0;
 #pragma unroll
@@ -216,7 +236,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
     for (uint32_t i = threadIdx.x; i < items; i += RS_THREADS) {
         const uint32_t key = s_keys[i];
         const uint32_t d = (key >> shift) & 255u;
-        const uint64_t dst = digit_offs[(uint64_t)d * nblocks + blockIdx.x] + (i - s_loc[d]);
+        const uint64_t dst = s_doff[d] + (i - s_loc[d]);
         keys_out[dst] = key;
         vals_out[dst] = s_vals[i];
     }
@@ -242,6 +262,8 @@ int radix_sort_pairs(uint32_t* keys, uint32_t* vals, const uint64_t* n_dev, uint
                      int key_bits, void* ws, hipStream_t s) {
     const int nb = radix_blocks(capacity);
     const uint64_t nh = 256ull * nb;
+    if (((uintptr_t)keys | (uintptr_t)ws) & 15)
+        return set_error(RF_ERR_INVALID, "radix_sort_pairs: keys and workspace must be 16-byte aligned");
     char* p = (char*)ws;
     uint32_t* k2 = (uint32_t*)p;
     uint32_t* v2 = k2 + capacity;
