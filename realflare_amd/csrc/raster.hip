@@ -706,8 +706,12 @@ __global__ __launch_bounds__(256) void k_count_heavy(const Setup* __restrict__ s
 
 // The quad steps k_expand deferred: per 64 rows, the rows' spans and prefix,
 // then the lanes write the chunk's pairs, pair j finding its row by binary
-// search over the prefix.  Output positions are fixed by the scan, so the
-// order the list was filled in does not matter.
+// search over the prefix; a chunk of more than 64 * HEAVY_ROW_WALK pairs is
+// written row by row instead (a search per 64 pairs costs more than a pass
+// over the rows there: a quad step over the whole screen has 65536 pairs).
+// Output positions are fixed by the scan, so the order the list was filled
+// in does not matter.
+static constexpr uint32_t HEAVY_ROW_WALK = 16;
 __global__ __launch_bounds__(256) void k_expand_heavy(const Setup* __restrict__ setup,
                                                       const uint64_t* __restrict__ offsets,
                                                       const Header* __restrict__ h, Geom G,
@@ -730,6 +734,23 @@ __global__ __launch_bounds__(256) void k_expand_heavy(const Setup* __restrict__ 
         const uint32_t* pp = s_pre[wave];
         for (uint32_t c = 0; c < rows; c += 64) {
             const uint32_t total = heavy_rows(rec, G, c, lane, s_a[wave], s_len[wave], s_pre[wave]);
+            if (total > 64 * HEAVY_ROW_WALK) {
+                // many pairs per row: row by row, each row's run of tiles spread
+                // over the lanes, no search
+                const uint32_t nrows = min(64u, rows - c);
+                for (uint32_t r = 0; r < nrows; r++) {
+                    const uint32_t len = pl[r];
+                    if (len == 0) continue;
+                    const uint64_t o = off + pp[r];
+                    const uint32_t key0 = frame_key + (ty0 + c + r) * (uint32_t)G.ntx + pa[r];
+                    for (uint32_t j = (uint32_t)lane; j < len; j += 64) {
+                        keys[o + j] = key0 + j;
+                        vals[o + j] = e;
+                    }
+                }
+                off += total;
+                continue;
+            }
             for (uint32_t j = (uint32_t)lane; j < total; j += 64) {
                 int lo = 0, hi = 63;  // last row whose prefix is <= j
                 while (lo < hi) {
