@@ -200,12 +200,22 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
         __syncthreads();
     }
     s_loc[threadIdx.x] -= cnt;
+    // every round's item is loaded up front, so the rounds do not each wait
+    // for memory
+    uint32_t kr[RS_ITEMS], vr[RS_ITEMS];
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; r++) {
+        const uint64_t idx = base + (uint64_t)r * RS_THREADS + threadIdx.x;
+        kr[r] = idx < n ? keys_in[idx] : 0u;
+        vr[r] = idx < n ? vals_in[idx] : 0u;
+    }
     __syncthreads();
+#pragma unroll
     for (int r = 0; r < RS_ITEMS; r++) {
         const uint64_t idx = base + (uint64_t)r * RS_THREADS + threadIdx.x;
         const bool valid = idx < n;
-        const uint32_t key = valid ? keys_in[idx] : 0u;
-        const uint32_t val = valid ? vals_in[idx] : 0u;
+        const uint32_t key = kr[r];
+        const uint32_t val = vr[r];
         const uint32_t d = (key >> shift) & 255u;
         uint64_t peers = __ballot(valid);
 #pragma unroll

@@ -138,7 +138,7 @@ static constexpr int SMALL_COORD = 1 << 21;
 // them would hold its wave for as many iterations.
 static constexpr uint32_t HEAVY_ROWS = 16;
 static constexpr uint32_t HEAVY_PAIRS = 1024;
-static constexpr int HEAVY_BLOCKS = 1024;  // persistent grids of the two kernels
+static constexpr int HEAVY_BLOCKS = 4096;  // persistent grids of the two kernels
 
 struct alignas(16) Setup {
     int vx[4], vy[4];
