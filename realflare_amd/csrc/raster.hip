@@ -486,7 +486,6 @@ __global__ __launch_bounds__(256) void k_setup(const float4* __restrict__ vtx,
         at.y = tab.z;
         at.z = tab.w;
         at.pad_ = 0.0f;
-        attr[e] = at;
         const bool collinear = edge_i(vx[0], vy[0], vx[1], vy[1], vx[2], vy[2]) == 0 &&
                                edge_i(vx[0], vy[0], vx[2], vy[2], vx[3], vy[3]) == 0;
         if (!collinear) {
@@ -558,11 +557,13 @@ __global__ __launch_bounds__(256) void k_setup(const float4* __restrict__ vtx,
                     }
                     er.flags = flags;
                     edges[e] = er;
+                    // the records of quad steps that bin no pair are never read
+                    setup[e] = rec;
+                    attr[e] = at;
                 }
             }
         }
     }
-    setup[e] = rec;
     counts[e] = count;
 }
 
