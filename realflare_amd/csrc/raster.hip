@@ -776,16 +776,34 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ p, long
     if (i < n) p[i] = v;
 }
 
+// Tile ranges of the sorted pairs: each thread looks at four consecutive keys
+// (one 16-byte load; the key array is 16-byte aligned) and the key after them.
 __global__ __launch_bounds__(256) void k_ranges(const uint32_t* __restrict__ keys,
                                                 const Header* __restrict__ h,
                                                 uint32_t* __restrict__ tstart,
                                                 uint32_t* __restrict__ tend) {
-    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const uint64_t i0 = 4 * ((uint64_t)blockIdx.x * 256 + threadIdx.x);
     const uint64_t n = h->n_sort;
-    if (i >= n) return;
-    const uint32_t k = keys[i];
-    if (i == 0 || keys[i - 1] != k) tstart[k] = (uint32_t)i;
-    if (i == n - 1 || keys[i + 1] != k) tend[k] = (uint32_t)(i + 1);
+    if (i0 >= n) return;
+    uint32_t k[5];
+    if (i0 + 4 <= n) {
+        const uint4 v = *reinterpret_cast<const uint4*>(keys + i0);
+        k[0] = v.x;
+        k[1] = v.y;
+        k[2] = v.z;
+        k[3] = v.w;
+    } else {
+        for (int j = 0; j < 4; j++) k[j] = i0 + j < n ? keys[i0 + j] : 0u;
+    }
+    k[4] = i0 + 4 < n ? keys[i0 + 4] : 0u;
+    const uint32_t prev = i0 > 0 ? keys[i0 - 1] : 0u;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const uint64_t i = i0 + j;
+        if (i >= n) break;
+        if (i == 0 || (j == 0 ? prev : k[j - 1]) != k[j]) tstart[k[j]] = (uint32_t)i;
+        if (i == n - 1 || k[j + 1] != k[j]) tend[k[j]] = (uint32_t)(i + 1);
+    }
 }
 
 // rasterizing.cl:55-93 with the edge biases precomputed per quad step.
@@ -1390,8 +1408,8 @@ static int bin_impl(const float4* vtx, const float4* bounds, const float* spectr
                        0u);
     hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(ntiles, 256)), dim3(256), 0, s, r.tend, ntiles, 0u);
     if (cap > 0) {
-        hipLaunchKernelGGL(k_ranges, dim3(blocks_for((long)cap, 256)), dim3(256), 0, s, r.keys, r.h,
-                           r.tstart, r.tend);
+        hipLaunchKernelGGL(k_ranges, dim3(blocks_for(((long)cap + 3) / 4, 256)), dim3(256), 0, s,
+                           r.keys, r.h, r.tstart, r.tend);
     }
     // the rasterizer's super-block order: largest pair count first (stable)
     const long nblk = super_blocks(G) * G.frames;
