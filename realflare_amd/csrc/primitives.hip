@@ -165,83 +165,106 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const uint32_t* __res
     hist[(uint64_t)threadIdx.x * nblocks + blockIdx.x] = c;
 }
 
-// Scatter of one 4096-item chunk: items are first ranked (stable: by digit,
-// then by position in the chunk) into LDS in digit order, then written out
-// from there, so consecutive threads write consecutive addresses of one
-// digit's run and the stores coalesce.  The chunk's digit counts come from
-// k_radix_hist (the same items, so the same counts).
+// Scatter of one 4096-item chunk.  Wave w owns the contiguous quarter
+// base + 1024 w ... base + 1024 w + 1023 and ranks it round by round (64
+// items a round, ranks within a round from eight ballots over the digit bits)
+// against its own digit counts in LDS, so the ranking synchronizes at wave
+// level only.  The four waves' counts then give each wave its place inside
+// each digit's run of the chunk, waves in order, so the sort stays stable;
+// the items are placed in LDS in digit order and written out from there, so
+// consecutive threads store consecutive addresses of one digit's run.  The
+// chunk's digit counts come from k_radix_hist (the same items, so the same
+// counts).
+static constexpr int RS_WAVES = RS_THREADS / 64;
+static constexpr int RS_WAVE_ITEMS = RS_CHUNK / RS_WAVES;  // 1024
+static_assert(RS_WAVE_ITEMS == 64 * RS_ITEMS, "one round of 64 items per thread item");
+
+__device__ __forceinline__ void rs_wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
     const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
     uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
     const uint64_t* __restrict__ n_ptr, int shift, int nblocks,
     const uint32_t* __restrict__ hist, const uint64_t* __restrict__ digit_offs) {
     __shared__ uint32_t s_keys[RS_CHUNK], s_vals[RS_CHUNK];
-    __shared__ uint32_t s_run[256];  // items of each digit placed so far
-    __shared__ uint32_t s_loc[256];  // first LDS slot of each digit
-    __shared__ uint32_t s_wcnt[RS_THREADS / 64][256];
-    __shared__ uint64_t s_doff[256];  // output offset of each digit's run of this chunk
+    __shared__ uint32_t s_loc[256];             // first LDS slot of each digit
+    __shared__ uint32_t s_wcnt[RS_WAVES][256];  // a wave's items per digit, then its offset in the run
+    __shared__ uint32_t s_wtot[RS_WAVES];
+    __shared__ uint64_t s_doff[256];            // output offset of each digit's run of this chunk
     const uint64_t n = *n_ptr;
     const uint64_t base = (uint64_t)blockIdx.x * RS_CHUNK;
     if (base >= n) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-    // exclusive scan of the chunk's digit counts (thread t owns digit t)
-    const uint32_t cnt = hist[(uint64_t)threadIdx.x * nblocks + blockIdx.x];
-    s_doff[threadIdx.x] = digit_offs[(uint64_t)threadIdx.x * nblocks + blockIdx.x];
-    s_loc[threadIdx.x] = cnt;
-    s_run[threadIdx.x] = 0;
-#pragma unroll
-    for (int w = 0; w < RS_THREADS / 64; w++) s_wcnt[w][threadIdx.x] = 0;
-    __syncthreads();
-    for (int off = 1; off < 256; off <<= 1) {
-        const uint32_t v = threadIdx.x >= (unsigned)off ? s_loc[threadIdx.x - off] : 0u;
-        __syncthreads();
-        s_loc[threadIdx.x] += v;
-        __syncthreads();
-    }
-    s_loc[threadIdx.x] -= cnt;
-    // every round's item is loaded up front, so the rounds do not each wait
-    // for memory
+    // the wave's items, loaded up front so the rounds do not each wait for memory
+    const uint64_t wbase = base + (uint64_t)wave * RS_WAVE_ITEMS;
     uint32_t kr[RS_ITEMS], vr[RS_ITEMS];
 #pragma unroll
     for (int r = 0; r < RS_ITEMS; r++) {
-        const uint64_t idx = base + (uint64_t)r * RS_THREADS + threadIdx.x;
+        const uint64_t idx = wbase + (uint64_t)r * 64 + lane;
         kr[r] = idx < n ? keys_in[idx] : 0u;
         vr[r] = idx < n ? vals_in[idx] : 0u;
     }
+    // thread t owns digit t: the chunk's count, the run's output offset
+    const uint32_t cnt = hist[(uint64_t)threadIdx.x * nblocks + blockIdx.x];
+    s_doff[threadIdx.x] = digit_offs[(uint64_t)threadIdx.x * nblocks + blockIdx.x];
+#pragma unroll
+    for (int w = 0; w < RS_WAVES; w++) s_wcnt[w][threadIdx.x] = 0;
+    uint32_t inc = cnt;  // inclusive scan of the counts over the wave's 64 digits
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t v = (uint32_t)__shfl_up((int)inc, off, 64);
+        if (lane >= off) inc += v;
+    }
+    if (lane == 63) s_wtot[wave] = inc;
     __syncthreads();
+    uint32_t below = 0;
+    for (int w = 0; w < wave; w++) below += s_wtot[w];
+    s_loc[threadIdx.x] = below + inc - cnt;
+    // ranks within the wave's quarter
+    uint32_t rank[RS_ITEMS];
+    uint32_t* wcnt = s_wcnt[wave];
 #pragma unroll
     for (int r = 0; r < RS_ITEMS; r++) {
-        const uint64_t idx = base + (uint64_t)r * RS_THREADS + threadIdx.x;
-        const bool valid = idx < n;
-        const uint32_t key = kr[r];
-        const uint32_t val = vr[r];
-        const uint32_t d = (key >> shift) & 255u;
+        const bool valid = wbase + (uint64_t)r * 64 + lane < n;
+        const uint32_t d = (kr[r] >> shift) & 255u;
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; b++) {
             const uint64_t m = __ballot(valid && ((d >> b) & 1u));
             peers &= ((d >> b) & 1u) ? m : ~m;
         }
-        const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
-        if (valid && rank == 0) s_wcnt[wave][d] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        if (valid) {
-            uint32_t pos = s_loc[d] + s_run[d] + rank;
-            for (int w = 0; w < wave; w++) pos += s_wcnt[w][d];
-            s_keys[pos] = key;
-            s_vals[pos] = val;
-        }
-        __syncthreads();
-        uint32_t add = 0;
-#pragma unroll
-        for (int w = 0; w < RS_THREADS / 64; w++) {
-            add += s_wcnt[w][threadIdx.x];
-            s_wcnt[w][threadIdx.x] = 0;
-        }
-        s_run[threadIdx.x] += add;
-        __syncthreads();
+        const uint32_t in_round = (uint32_t)__popcll(peers & lt_mask);
+        const uint32_t before = wcnt[d];
+        rank[r] = before + in_round;
+        rs_wave_sync();  // every lane has read its digit's count
+        if (valid && in_round == 0) wcnt[d] = before + (uint32_t)__popcll(peers);
+        rs_wave_sync();
     }
+    __syncthreads();
+    {  // each wave's offset inside each digit's run, waves in order
+        uint32_t acc = 0;
+#pragma unroll
+        for (int w = 0; w < RS_WAVES; w++) {
+            const uint32_t c = s_wcnt[w][threadIdx.x];
+            s_wcnt[w][threadIdx.x] = acc;
+            acc += c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; r++) {
+        if (wbase + (uint64_t)r * 64 + lane < n) {
+            const uint32_t d = (kr[r] >> shift) & 255u;
+            const uint32_t pos = s_loc[d] + wcnt[d] + rank[r];
+            s_keys[pos] = kr[r];
+            s_vals[pos] = vr[r];
+        }
+    }
+    __syncthreads();
     const uint32_t items = (uint32_t)min((uint64_t)RS_CHUNK, n - base);
     for (uint32_t i = threadIdx.x; i < items; i += RS_THREADS) {
         const uint32_t key = s_keys[i];
