@@ -31,3 +31,31 @@ def test_invalid_arguments_report_error_without_gpu():
     rc = fn(None, None, None, ctypes.byref(prm), None, None)
     assert rc == native.RF_ERR_INVALID
     assert 'lens_count' in native.last_error()
+
+
+def test_raster_ghost_texture_limits_without_gpu():
+    """The rasterizer reads the ghost texture through a zero-framed copy with
+    24-bit texel arithmetic: each side must stay below 2^24 - 4 texels (and the
+    texture at most 2^28 texels); the size query answers 0 and the entry point
+    RF_ERR_INVALID beyond that."""
+    from realflare_amd import native
+    from realflare_amd.api.tasks.rasterizing import raster_params
+
+    lib = native.lib()
+    size = lib.rf_rasterize_workspace_size
+    size.restype = ctypes.c_size_t
+    prm = raster_params(2, 2, 4, 1.0, (64, 64), 64, 1, 1, (36.0, 24.0), 0.0, 1.0, 8.0, (16, 16), 4)
+    ok = size(ctypes.byref(prm), ctypes.c_uint64(1024))
+    assert ok > 0
+    # one more texel per side grows the workspace (the framed copy is in it)
+    prm.ghost_width, prm.ghost_height = 4096, 4096
+    assert size(ctypes.byref(prm), ctypes.c_uint64(1024)) >= ok + 4096 * 4096 * 4
+    prm.ghost_width, prm.ghost_height = (1 << 24) - 5, 1
+    assert size(ctypes.byref(prm), ctypes.c_uint64(1024)) > 0
+    prm.ghost_width = (1 << 24) - 4
+    assert size(ctypes.byref(prm), ctypes.c_uint64(1024)) == 0
+    fn = lib.rf_raster_bin
+    fn.restype = ctypes.c_int
+    rc = fn(None, None, None, ctypes.byref(prm), ctypes.c_uint64(0), None, ctypes.c_size_t(0), None)
+    assert rc == native.RF_ERR_INVALID
+    assert 'ghost texture side' in native.last_error()
