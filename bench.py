@@ -359,7 +359,7 @@ def fit_timing(timed_steps: int = 3, fit_steps: int = 100) -> dict:
     best_params, hist = fit(fs, start, fit_steps, robust=0.9)
     losses_fit = [h['loss'] for h in hist]
     huber = fs.huber_delta
-    # judged by the squared error, at the start and at the best points
+    # measured by the squared error, at the start and at the best points
     fs.huber_delta = 0.0
     first = [float(x) for x in fs.step(start)[0]]
     best = [float(x) for x in fs.step(best_params)[0]]
