@@ -831,6 +831,18 @@ __device__ __forceinline__ int edge_row(const Edges& er, int k, int px, int py) 
     return (int)((uint32_t)__mul24(er.a[k], py) + (uint32_t)__mul24(er.nb[k], px) + (uint32_t)er.c[k]);
 }
 
+// The front / back test of rasterizing.cl:55-93 over the signs w_k of the edges
+// 01, 12, 23, 30, 20 (bit masks or bools): front = w20 w01 w12 (w23 | w30) |
+// !w20 w23 w30 (w01 | w12), back = !w20 !w01 !w12 (!w23 | !w30) | w20 !w23
+// !w30 (!w01 | !w12).  With A = w01 w12, O = w01 | w12, B = w23 w30,
+// P = w23 | w30, front | back = w20 (A P | !A !P) | !w20 (B O | !B !O): an
+// equality of A and P where w20 holds, of B and O where it does not.
+template <typename T>
+__device__ __forceinline__ T cover_terms(T w01, T w12, T w23, T w30, T w20) {
+    const T a = w01 & w12, o = w01 | w12, b = w23 & w30, p = w23 | w30;
+    return (w20 & ~(a ^ p)) | (~w20 & ~(b ^ o));
+}
+
 // cover_small for every lane of the wave at once, as a lane mask: one
 // comparison per edge gives the edge's sign bits of all 64 samples, and the
 // front / back tests combine the five masks with wave-uniform (scalar) bit
@@ -841,20 +853,14 @@ __device__ __forceinline__ uint64_t cover_small_mask(int px, int py, const Edges
     const uint64_t w23 = __ballot(edge_row(er, 2, px, py) >= 0);
     const uint64_t w30 = __ballot(edge_row(er, 3, px, py) >= 0);
     const uint64_t w20 = __ballot(edge_row(er, 4, px, py) >= 0);
-    const uint64_t front = (w20 & w01 & w12 & (w23 | w30)) | (~w20 & w23 & w30 & (w01 | w12));
-    const uint64_t back = (~w20 & ~w01 & ~w12 & (~w23 | ~w30)) | (w20 & ~w23 & ~w30 & (~w01 | ~w12));
-    return front | back;
+    return cover_terms(w01, w12, w23, w30, w20);
 }
 
 __device__ __forceinline__ bool cover_small(int px, int py, const Edges& er) {
-    const bool w01 = edge_row(er, 0, px, py) >= 0;
-    const bool w12 = edge_row(er, 1, px, py) >= 0;
-    const bool w23 = edge_row(er, 2, px, py) >= 0;
-    const bool w30 = edge_row(er, 3, px, py) >= 0;
-    const bool w20 = edge_row(er, 4, px, py) >= 0;
-    const bool front = (w20 && w01 && w12 && (w23 || w30)) || (!w20 && w23 && w30 && (w01 || w12));
-    const bool back = (!w20 && !w01 && !w12 && (!w23 || !w30)) || (w20 && !w23 && !w30 && (!w01 || !w12));
-    return front || back;
+    const uint32_t w01 = edge_row(er, 0, px, py) >= 0, w12 = edge_row(er, 1, px, py) >= 0;
+    const uint32_t w23 = edge_row(er, 2, px, py) >= 0, w30 = edge_row(er, 3, px, py) >= 0;
+    const uint32_t w20 = edge_row(er, 4, px, py) >= 0;
+    return cover_terms(w01, w12, w23, w30, w20) & 1u;
 }
 
 // rasterizing.cl:95-148 (mean value coordinates of a quad), shading precision:
