@@ -1141,33 +1141,34 @@ __global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ bounds
     const uint32_t start = tstart[tile], end = tend[tile];
     const uint64_t inside_mask = __ballot(inside);
     // The pair ids are read 64 at a time by one vector load (lane l holds the
-    // id of pair start + 64 b + l) and taken per pair with readlane, so no
-    // scalar load of an id precedes the rows of a pair.  The scalar load of a
-    // pair's edge rows is then the walk's only latency: once the rows of pair
-    // k have arrived, one dword of the rows of pair k + TOUCH is loaded, which
-    // brings that pair's 64-byte record into the scalar cache while pairs
-    // k .. k + TOUCH - 1 are processed.  Its address depends on the rows just
-    // loaded: scalar loads complete out of order, so a wait covers every one
-    // in flight, and a touch issued before the rows' wait would be waited for
-    // with them.  The touched value is consumed by an empty asm statement
-    // after the wait of a later pair, so the load is kept and costs no wait.
-    uint32_t ids = 0, touched = 0;
-    for (uint32_t k = start; k < end; k++) {
-        const uint32_t j = (k - start) & 63;
-        if (j == 0) ids = k + (uint32_t)lane < end ? vals[k + lane] : 0u;
+    // id of pair kb + l) and taken per pair with readlane, so no scalar load
+    // of an id precedes the rows of a pair.  The scalar load of a pair's edge
+    // rows is then the walk's only latency: once the rows of pair j have
+    // arrived, one dword of the rows of pair j + TOUCH is loaded, which brings
+    // that pair's 64-byte record into the scalar cache while pairs
+    // j .. j + TOUCH - 1 are processed.  The touch sits in the branch on the
+    // rows' flags, so it is issued after the rows' wait: scalar loads complete
+    // out of order, so a wait covers every one in flight, and a touch issued
+    // before that wait would be waited for with the rows.  The touched value
+    // is consumed by an empty asm statement after the wait of a later pair, so
+    // the load is kept and costs no wait.
+    uint32_t touched = 0;
+    for (uint32_t kb = start; kb < end; kb += 64) {
+    const uint32_t ids = kb + (uint32_t)lane < end ? vals[kb + lane] : 0u;
+    const uint32_t jn = (uint32_t)__builtin_amdgcn_readfirstlane((int)min(64u, end - kb));
+    for (uint32_t j = 0; j < jn; j++) {
         const uint32_t e = (uint32_t)__builtin_amdgcn_readlane((int)ids, (int)j);
         const Edges er = edges[e];
-        if (j < 64 - TOUCH && k + TOUCH < end) {
-            const uint32_t en = (uint32_t)__builtin_amdgcn_readlane((int)ids, (int)(j + TOUCH)) ^
-                                ((uint32_t)er.flags >> 31);  // flags < 64: en is the id itself
-            asm volatile("" ::"s"(touched));
-            touched = reinterpret_cast<const uint32_t*>(edges)[(size_t)en * (sizeof(Edges) / 4)];
-        }
         int hits = 0;
         uint64_t m;
         // one sample per pixel and bins made of whole tiles: the coverage of a
         // FLAG_SMALL quad step is computed as a lane mask (cover_small_mask)
         if (kMask && (er.flags & FLAG_SMALL)) {
+            if (j + TOUCH < jn) {
+                const uint32_t en = (uint32_t)__builtin_amdgcn_readlane((int)ids, (int)(j + TOUCH));
+                asm volatile("" ::"s"(touched));
+                touched = reinterpret_cast<const uint32_t*>(edges)[(size_t)en * (sizeof(Edges) / 4)];
+            }
             m = cover_small_mask(px, py, er) & inside_mask;
             hits = 1;  // read by the covered lanes only
         } else {
@@ -1207,6 +1208,7 @@ __global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ bounds
         if (lane == 0) s_slot[nslot] = make_uint4((uint32_t)m, (uint32_t)(m >> 32), qlen, e);
         qlen += n;
         nslot++;
+    }
     }
     if (qlen > 0) flush();
     if (!inside) return;
