@@ -460,32 +460,20 @@ __global__ __launch_bounds__(256) void k_setup(const float4* __restrict__ vtx,
         const float4 tab = table[step];
         const float blend = tab.x;
         int vx[4], vy[4];
-        Attr at;
+        long vbase[4];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const long base = l.ray0 + (long)qv[j] * G.W + wl;
-            const float4 a0 = vtx[2 * base], a1 = vtx[2 * base + 1];
-            float4 b0 = make_float4(0, 0, 0, 0), b1 = make_float4(0, 0, 0, 0);
-            if (base + 1 < G.NV) {
-                b0 = vtx[2 * (base + 1)];
-                b1 = vtx[2 * (base + 1) + 1];
-            }
+            vbase[j] = base;
+            const float4 a0 = vtx[2 * base];
+            const float4 b0 = base + 1 < G.NV ? vtx[2 * (base + 1)] : make_float4(0, 0, 0, 0);
             const float mx = a0.x + (b0.x - a0.x) * blend;
             const float my = a0.y + (b0.y - a0.y) * blend;
             vx[j] = rf_to_int(mx * (float)G.AA);
             vy[j] = rf_to_int(my * (float)G.AA);
             rec.vx[j] = vx[j];
             rec.vy[j] = vy[j];
-            at.u[j] = a0.z + (b0.z - a0.z) * blend;
-            at.v[j] = a0.w + (b0.w - a0.w) * blend;
-            at.rrel[j] = a1.x + (b1.x - a1.x) * blend;
-            at.refl[j] = a1.y + (b1.y - a1.y) * blend;
-            at.inten[j] = a1.z + (b1.z - a1.z) * blend;
         }
-        at.x = tab.y;
-        at.y = tab.z;
-        at.z = tab.w;
-        at.pad_ = 0.0f;
         const bool collinear = edge_i(vx[0], vy[0], vx[1], vy[1], vx[2], vy[2]) == 0 &&
                                edge_i(vx[0], vy[0], vx[2], vy[2], vx[3], vy[3]) == 0;
         if (!collinear) {
@@ -557,8 +545,30 @@ __global__ __launch_bounds__(256) void k_setup(const float4* __restrict__ vtx,
                     }
                     er.flags = flags;
                     edges[e] = er;
-                    // the records of quad steps that bin no pair are never read
+                    // the records of quad steps that bin no pair are never read;
+                    // the shading inputs are blended here, from vertices still
+                    // in cache, so they are not held through the counting above
                     setup[e] = rec;
+                    Attr at;
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const long base = vbase[j];
+                        const float4 a0 = vtx[2 * base], a1 = vtx[2 * base + 1];
+                        float4 b0 = make_float4(0, 0, 0, 0), b1 = make_float4(0, 0, 0, 0);
+                        if (base + 1 < G.NV) {
+                            b0 = vtx[2 * (base + 1)];
+                            b1 = vtx[2 * (base + 1) + 1];
+                        }
+                        at.u[j] = a0.z + (b0.z - a0.z) * blend;
+                        at.v[j] = a0.w + (b0.w - a0.w) * blend;
+                        at.rrel[j] = a1.x + (b1.x - a1.x) * blend;
+                        at.refl[j] = a1.y + (b1.y - a1.y) * blend;
+                        at.inten[j] = a1.z + (b1.z - a1.z) * blend;
+                    }
+                    at.x = tab.y;
+                    at.y = tab.z;
+                    at.z = tab.w;
+                    at.pad_ = 0.0f;
                     attr[e] = at;
                 }
             }
