@@ -53,6 +53,8 @@ struct Geom {
     // + 1) / 2 of rasterizing.cl and the texel-centre offset; gwf, ghf = the
     // texture size as floats (the upper clamp of the lookup).
     float gku, gcu, gkv, gcv, gwf, ghf;
+    int aa_shift;   // log2(AA)
+    int bin_shift;  // log2(bin) when bin is a power of two, else -1
 };
 
 // Where a ghost's data lives: first ray (planes [W][R], vertices [R][W]),
@@ -175,6 +177,10 @@ __host__ __device__ __forceinline__ int sub_offset(int i) { return (int)((SUB_OF
 
 __device__ __forceinline__ long floordiv(long a, long b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
 __device__ __forceinline__ long ceildiv(long a, long b) { return -floordiv(-a, b); }
+// floor(a / 2^s) for either sign (an arithmetic shift)
+__device__ __forceinline__ long floorshr(long a, int s) { return a >> s; }
+static constexpr int TILE_SHIFT = 3;
+static_assert(TILE == 1 << TILE_SHIFT, "tile edge is a power of two");
 
 __device__ __forceinline__ void quad_vertexes(int g, int quad, int q[4]) {
     int row = quad / (g - 1), col = quad % (g - 1);
@@ -378,6 +384,17 @@ struct HullRows {
 __host__ __device__ constexpr int hull_pa(int s) { return s < 3 ? 0 : (s < 5 ? 1 : 2); }
 __host__ __device__ constexpr int hull_pb(int s) { return s < 3 ? s + 1 : (s < 5 ? s - 1 : 3); }
 
+// n / d to within an ulp or two: the hardware reciprocal, one Newton step and
+// a residual correction (the spans are widened by a sample, so the quotient
+// need not be the correctly rounded one; every kernel that counts or lists a
+// quad step's tiles uses this same function, so they agree).
+__device__ __forceinline__ double hull_quot(double n, double d) {
+    double r = __builtin_amdgcn_rcp(d);
+    r = fma(fma(-d, r, 1.0), r, r);
+    const double q = n * r;
+    return fma(fma(-d, q, n), r, q);
+}
+
 __device__ __forceinline__ void hull_prepare(const int* vx, const int* vy, HullRows* h) {
 #pragma unroll
     for (int i = 0; i < 4; i++) {
@@ -388,12 +405,14 @@ __device__ __forceinline__ void hull_prepare(const int* vx, const int* vy, HullR
     for (int s = 0; s < 6; s++) {
         const int i = hull_pa(s), j = hull_pb(s);
         h->valid[s] = vy[i] != vy[j];
-        h->slope[s] = h->valid[s] ? ((double)vx[j] - vx[i]) / ((double)vy[j] - vy[i]) : 0.0;
+        h->slope[s] = h->valid[s] ? hull_quot((double)vx[j] - vx[i], (double)vy[j] - vy[i]) : 0.0;
     }
 }
 
-// Tiles [*a, *b] of tile row `ty` the hull can reach (empty when *a > *b).
-__device__ __forceinline__ void hull_row_span(const HullRows& h, int AA, int ty, long* a, long* b) {
+// Tiles [*a, *b] of tile row `ty` the hull can reach (empty when *a > *b);
+// aa_shift = log2(AA).
+__device__ __forceinline__ void hull_row_span(const HullRows& h, int AA, int aa_shift, int ty, long* a,
+                                              long* b) {
     const double Y0 = (double)ty * TILE * AA, Y1 = ((double)ty + 1) * TILE * AA - 1;
     double xmin = INFINITY, xmax = -INFINITY;
 #pragma unroll
@@ -425,8 +444,8 @@ __device__ __forceinline__ void hull_row_span(const HullRows& h, int AA, int ty,
     const double lim = 4.0e18;
     const long sl = (long)fmax(fmin(floor(xmin) - 1, lim), -lim);
     const long sh = (long)fmax(fmin(ceil(xmax) + 1, lim), -lim);
-    *a = floordiv(floordiv(sl, AA), TILE);
-    *b = floordiv(floordiv(sh, AA), TILE);
+    *a = floorshr(floorshr(sl, aa_shift), TILE_SHIFT);
+    *b = floorshr(floorshr(sh, aa_shift), TILE_SHIFT);
 }
 
 // Per (quad, step): integer vertices, edge biases, tile rectangle, pair
@@ -494,15 +513,25 @@ __global__ __launch_bounds__(256) void k_setup(const float4* __restrict__ vtx,
             long sx1 = max(max(vx[0], vx[1]), max(vx[2], vx[3]));
             long sy0 = min(min(vy[0], vy[1]), min(vy[2], vy[3]));
             long sy1 = max(max(vy[0], vy[1]), max(vy[2], vy[3]));
-            long x0 = floordiv(sx0, G.AA), x1 = floordiv(sx1, G.AA);
-            long y0 = floordiv(sy0, G.AA), y1 = floordiv(sy1, G.AA);
+            long x0 = floorshr(sx0, G.aa_shift), x1 = floorshr(sx1, G.aa_shift);
+            long y0 = floorshr(sy0, G.aa_shift), y1 = floorshr(sy1, G.aa_shift);
             // bins the binner lists this quad in (rasterizing.cl:449-453)
             const int sbx0 = wadd(rf_to_int(b.x * G.st), G.width / 2);
             const int sby0 = wadd(rf_to_int(b.y * G.st), G.height / 2);
             const int sbx1 = wadd(rf_to_int(b.z * G.st), G.width / 2);
             const int sby1 = wadd(rf_to_int(b.w * G.st), G.height / 2);
-            long bx0 = ceildiv((long)sbx0 - G.bin, G.bin), bx1 = floordiv(sbx1, G.bin);
-            long by0 = ceildiv((long)sby0 - G.bin, G.bin), by1 = floordiv(sby1, G.bin);
+            long bx0, bx1, by0, by1;
+            if (G.bin_shift >= 0) {  // ceil(a / bin) = -floor(-a / bin)
+                bx0 = -floorshr(-((long)sbx0 - G.bin), G.bin_shift);
+                bx1 = floorshr(sbx1, G.bin_shift);
+                by0 = -floorshr(-((long)sby0 - G.bin), G.bin_shift);
+                by1 = floorshr(sby1, G.bin_shift);
+            } else {
+                bx0 = ceildiv((long)sbx0 - G.bin, G.bin);
+                bx1 = floordiv(sbx1, G.bin);
+                by0 = ceildiv((long)sby0 - G.bin, G.bin);
+                by1 = floordiv(sby1, G.bin);
+            }
             bx0 = max(bx0, 0L);
             by0 = max(by0, 0L);
             bx1 = min(bx1, (long)G.nbx - 1);
@@ -524,7 +553,7 @@ __global__ __launch_bounds__(256) void k_setup(const float4* __restrict__ vtx,
                     hull_prepare(vx, vy, &hull);
                     for (uint32_t ty = ty0; ty <= ty1; ty++) {
                         long a, bb;
-                        hull_row_span(hull, G.AA, (int)ty, &a, &bb);
+                        hull_row_span(hull, G.AA, G.aa_shift, (int)ty, &a, &bb);
                         a = max(a, (long)tx0);
                         bb = min(bb, (long)tx1);
                         if (a <= bb) count += (uint32_t)(bb - a + 1);
@@ -603,7 +632,7 @@ __device__ __forceinline__ void expand_entry(const Setup& rec, const Geom& G, ui
         G.batch ? (uint32_t)loc_quad(G, rec.prim).frame * (uint32_t)(G.ntx * G.nty) : 0u;
     for (uint32_t ty = ty0; ty <= ty1; ty++) {
         long a, b;
-        hull_row_span(hull, G.AA, (int)ty, &a, &b);
+        hull_row_span(hull, G.AA, G.aa_shift, (int)ty, &a, &b);
         a = max(a, (long)tx0);
         b = min(b, (long)tx1);
         for (long tx = a; tx <= b; tx++) {
@@ -673,7 +702,7 @@ __device__ __forceinline__ uint32_t heavy_rows(const Setup& rec, const Geom& G, 
         HullRows hull;
         hull_prepare(rec.vx, rec.vy, &hull);
         long a, b;
-        hull_row_span(hull, G.AA, (int)ty, &a, &b);
+        hull_row_span(hull, G.AA, G.aa_shift, (int)ty, &a, &b);
         a = max(a, (long)tx0);
         b = min(b, (long)tx1);
         if (a <= b) {
@@ -1310,6 +1339,13 @@ static int make_geom(const rf_raster_params* p, Geom* G) {
     G->gcv = 0.5f * (float)G->gh - 0.5f;
     G->gwf = (float)G->gw;
     G->ghf = (float)G->gh;
+    G->aa_shift = 0;
+    while ((1 << G->aa_shift) < G->AA) G->aa_shift++;
+    G->bin_shift = -1;
+    if ((G->bin & (G->bin - 1)) == 0) {
+        G->bin_shift = 0;
+        while ((1 << G->bin_shift) < G->bin) G->bin_shift++;
+    }
     if (G->W > 1 && G->nspec < 1)
         return set_error(RF_ERR_INVALID, "raster: spectrum_count must be >= 1");
     if (G->NE > 0xffffffffL)
