@@ -389,6 +389,8 @@ def main() -> int:
     ap.add_argument('--warmup', type=int, default=3)
     ap.add_argument('--workload', default='full', choices=['full', 'small'])
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--no-graph', action='store_true',
+                    help='time frames launched one by one instead of graph replays')
     ap.add_argument('--no-batched', action='store_true', help='skip the batched configuration')
     ap.add_argument('--batch-frames', type=int, default=64)
     ap.add_argument('--no-fit', action='store_true', help='skip the batched fit configuration')
@@ -417,13 +419,29 @@ def main() -> int:
         pipe.render()
     torch.cuda.synchronize()
 
-    # timed region: exactly `steps` frames
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    # frames launched one by one (about 40 launches each), for reference
+    torch.cuda.synchronize()
+    start.record()
+    for _ in range(args.steps):
+        pipe.render()
+    stop.record()
+    torch.cuda.synchronize()
+    eager_ms = start.elapsed_time(stop) / args.steps
+
+    # timed region: exactly `steps` frames, each one replay of the frame
+    # captured as a HIP graph (the same launches, without their host-side gaps)
+    graph = None
+    if not args.no_graph:
+        graph = pipe.capture()
+        for _ in range(max(1, args.warmup)):
+            graph.replay()
+    frame = graph.replay if graph is not None else pipe.render
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     start.record()
     for _ in range(args.steps):
-        pipe.render()
+        frame()
     stop.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -467,6 +485,8 @@ def main() -> int:
         'warmup': args.warmup,
         'ms_per_step': ms,
         'frames_per_s': 1000.0 / ms,
+        'frame_launch': 'hip graph replay' if graph is not None else 'eager',
+        'eager_ms_per_step': eager_ms,
         'higher_is_better': True,
         'scaling': 'weak',
         'vs_baseline': None,

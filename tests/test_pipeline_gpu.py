@@ -40,3 +40,23 @@ def test_two_stream_frame_equals_serial_and_engine():
     err = np.abs(got - ref).sum() / np.abs(ref).sum()
     print(f'pipeline vs engine rel L1 {err:.2e}', flush=True)
     assert err <= 1e-6
+
+
+def test_captured_frame_replays_equal_eager_frame():
+    """The frame captured as a HIP graph (both streams): every replay, with
+    eager frames and host synchronizations between them, reproduces the eager
+    frame exactly, including after the output has been overwritten."""
+    wl = workload('small')
+    pipe = FramePipeline(wl.project, wl.lens_model, wl.paths, wl.wavelengths)
+    pipe.fit_capacity()
+    ref = pipe.render().clone()
+    torch.cuda.synchronize()
+    graph = pipe.capture()
+    for i in range(3):
+        pipe.out.fill_(float('nan'))
+        graph.replay()
+        torch.cuda.synchronize()
+        print(f'replay {i}', flush=True)
+        assert torch.equal(pipe.out, ref)
+        assert torch.equal(pipe.render(), ref)  # eager frame between replays
+        torch.cuda.synchronize()
