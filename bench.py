@@ -399,7 +399,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py measures one GPU; --gpus {args.gpus} is not supported (the flare path '
-              'has no multi-GPU sharding in this build)', file=sys.stderr)
+              'runs on one GPU; frames or ghosts are not sharded across GPUs)', file=sys.stderr)
         return 2
 
     import torch

@@ -1,5 +1,5 @@
 """Image-based light (Engine.image_flare) against the reference's own
-procedure (engine.py:98-155) run with this build's plain path: for every
+procedure (engine.py:98-155) run with the package's plain path: for every
 non-zero sample of the top-left quadrant, the plain FLARE render with the light
 at that sample, added with its four RGBA weights as rendered and mirrored, then
 divided by the sample count.  Same fragments, summed in the same order per
