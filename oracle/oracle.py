@@ -15,12 +15,10 @@ the render path in ``realflare_amd`` never does.
 * The lens diagram kernels (``realflare/api/tasks/cl/diagram.cl``) are
   restated in float32 numpy, vectorized over pixels.
 
-Pinning: ``tests/test_oracle.py`` and ``tests/test_host.py`` check these
-functions against the reference's test vectors, closed forms, invariants,
-brute force or golden fixtures; the header of each test names which.  Parity
-unpinned (no reference fixture or closed form checks them): ``starburst``, the
-grating / scratches / dust aperture masks, and the prim / vertex shaders
-beyond what the rasterizer checks exercise.
+Pinning: ``tests/test_oracle.py``, ``tests/test_oracle_pins.py`` and
+``tests/test_host.py`` check these functions against the reference's test
+vectors, closed forms, invariants, brute force or golden fixtures; the header
+of each test names which.
 """
 from __future__ import annotations
 

@@ -7,10 +7,11 @@
  * path links it.
  *
  * Each function cites the reference source it restates (paths relative to
- * the realflare repository).  Parity unpinned (no reference fixture or closed
- * form checks them): starburst, the grating / scratches / dust masks, and
- * prim_shader / vertex_shader beyond what the rasterizer checks exercise.  Behaviour the reference leaves undefined is
- * fixed here, identically to the HIP kernels:
+ * the realflare repository).  Every function is pinned by a check that does
+ * not restate it (tests/test_oracle.py, tests/test_oracle_pins.py): the
+ * reference's own vectors, closed forms, invariants or brute force.
+ * Behaviour the reference leaves undefined is fixed here, identically to the
+ * HIP kernels:
  *   - float -> int conversion truncates toward zero, saturates, NaN -> 0;
  *   - integer edge functions wrap modulo 2^32;
  *   - max/min/clamp on floats are IEEE maxNum/minNum (NaN loses);
