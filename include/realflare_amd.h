@@ -287,8 +287,9 @@ int rf_vertex_shader(const float *planes, const float *intensities, const rf_ras
  *   image           float[height][width][4], fully written
  * pair_capacity bounds the (8x8 tile, quad step) pairs a frame may bin; the
  * workspace (rf_rasterize_workspace_size) grows with it.  If a frame needs
- * more, nothing is drawn and rf_raster_status reports overflow and the count
- * needed; call again with a larger capacity.
+ * more, the image is cleared to 0 (left as it was when accumulating) and
+ * rf_raster_status reports overflow and the count needed; call again with a
+ * larger capacity.
  */
 size_t rf_rasterize_workspace_size(const rf_raster_params *prm, uint64_t pair_capacity);
 int rf_rasterize(const void *vertexes, const float *bounds, const float *ghost,

@@ -1125,11 +1125,19 @@ __global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ bounds
     __shared__ uint32_t s_q[QCAP];  // pixel lane | hits << 6 | slot << 10
     __shared__ uint4 s_slot[QCAP];  // per queued pair: coverage mask lo, hi, first queue index, entry
     __shared__ float4 s_c[QCAP];    // shaded colour of each queue entry
-    if (h->overflow) return;
+    const int lane = threadIdx.x;
+    if (h->overflow) {
+        // the pairs were not binned: the image is cleared (an accumulated
+        // image is left as it was), never left holding an earlier frame
+        if (G.accumulate) return;
+        const long px = (long)G.width * G.height * G.frames;
+        for (long i = (long)blockIdx.x * 64 + lane; i < px; i += (long)gridDim.x * 64)
+            image[i] = make_float4(0, 0, 0, 0);
+        return;
+    }
     int frame, tx, ty;
     if (!raster_tile(blockIdx.x, G, order, &frame, &tx, &ty)) return;
-    const int tile = frame * G.ntx * G.nty + ty * G.ntx + tx;
-    const int lane = threadIdx.x;
+    const uint32_t tile = (uint32_t)frame * (uint32_t)(G.ntx * G.nty) + (uint32_t)(ty * G.ntx + tx);
     const int x = tx * TILE + (lane & (TILE - 1)), y = ty * TILE + (lane / TILE);
     const bool inside = x < G.width && y < G.height;
     const int AA = kMask ? 1 : G.AA;
@@ -1762,3 +1770,4 @@ extern "C" int rf_raster_status(const void* workspace, uint64_t* required_pairs,
     *overflow = (int)h.overflow;
     return RF_OK;
 }
+

@@ -6,9 +6,13 @@ ghost aperture -> ghost FrFT -> trace -> prim/vertex shaders -> binner ->
 rasterizer, starburst aperture -> power spectrum -> starburst, and the
 composition ``flare + (flare + starburst)``.  All launches go to torch's
 current stream and nothing synchronizes with the host inside ``render``, so
-a frame can be timed with events, replayed per light position, or captured
-into a HIP graph.  Ghost culling is not run: it is the identity at
-cull_percentage 0 (preprocessing.py:80-83), which the standard workloads use.
+a frame can be timed with events, rendered again after ``set_light``, or
+captured into a HIP graph.  A frame whose binning pairs exceed the pair
+capacity renders a cleared flare (never an earlier frame's) and
+``raster_status`` reports it; once ``fit_capacity`` has sized the capacity,
+``set_light`` re-checks it for the new light and grows it when needed.
+Ghost culling is not run: it is the identity at cull_percentage 0
+(preprocessing.py:80-83), which the standard workloads use.
 """
 from __future__ import annotations
 
@@ -81,6 +85,7 @@ class FramePipeline:
             render.wavelength_sub_count, self.sensor, lens.min_area, project.flare.light.intensity,
             lens.fstop, (gres[1], gres[0]), self.spectrum.shape[0])
         self.pair_capacity = int(pair_capacity or max(1 << 22, 8 * P * self.Q))
+        self._fitted = False
         self._alloc_raster_ws()
         w, h = self.resolution
         self.flare = torch.empty((h, w, 4), dtype=torch.float32, device=dev)
@@ -105,7 +110,11 @@ class FramePipeline:
         self.raster_ws = torch.empty((n,), dtype=torch.uint8, device=self.device)
 
     def set_light(self, x: float, y: float) -> None:
-        """Light position in normalized screen coordinates (Flare.Light.position)."""
+        """Light position in normalized screen coordinates (Flare.Light.position).
+        After ``fit_capacity``, the frame at the new position is rendered once
+        and the pair capacity grown if that frame needs more (a graph captured
+        before must be captured again in any case: its launch parameters are
+        the old light's)."""
         lens = self.project.flare.lens
         direction = light_direction((x, y), self.resolution, self.sensor,
                                     self.lens_model.focal_length)
@@ -121,6 +130,8 @@ class FramePipeline:
         self.star_prm = starburst_params(
             flare.starburst, self.resolution, self.project.render.starburst.samples, (x, y),
             (flare.starburst.scale.width(), flare.starburst.scale.height()))
+        if self._fitted:
+            self.fit_capacity()
 
     # -- stages -----------------------------------------------------------------
     def stage_fns(self):
@@ -246,7 +257,9 @@ class FramePipeline:
         return int(req.value), bool(ovf.value)
 
     def fit_capacity(self) -> int:
-        """Render once and size the pair capacity to what this frame needs."""
+        """Render once and grow the pair capacity if this frame needs more
+        (then render it again); later ``set_light`` calls re-check it."""
+        self._fitted = True
         self.render()
         required, overflow = self.raster_status()
         if overflow or required > self.pair_capacity:
@@ -256,5 +269,7 @@ class FramePipeline:
             self._alloc_raster_ws()
             self.render()
             required, overflow = self.raster_status()
-            assert not overflow
+            if overflow:
+                raise RuntimeError(f'binning needs {required} pairs beyond the capacity '
+                                   f'{self.pair_capacity} it was just sized to')
         return required

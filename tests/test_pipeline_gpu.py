@@ -60,3 +60,53 @@ def test_captured_frame_replays_equal_eager_frame():
         assert torch.equal(pipe.out, ref)
         assert torch.equal(pipe.render(), ref)  # eager frame between replays
         torch.cuda.synchronize()
+
+
+def test_light_move_beyond_capacity_never_shows_the_previous_frame():
+    """A light move whose frame needs more binning pairs than the capacity:
+    without a fitted capacity the frame's flare is cleared and raster_status
+    reports the overflow; with one, set_light grows it and the frame equals
+    a fresh pipeline's render at the new light.  Neither returns the
+    previous light's image."""
+    wl = workload('small')
+    probe = FramePipeline(wl.project, wl.lens_model, wl.paths, wl.wavelengths,
+                          pair_capacity=1 << 24)
+    needs = {}
+    for pos in ((0.3, 0.2), (0.0, 0.0), (-0.6, 0.5), (0.8, -0.7)):
+        probe.set_light(*pos)
+        probe.render()
+        needs[pos] = probe.raster_status()[0]
+    lo = min(needs, key=needs.get)
+    hi = max(needs, key=needs.get)
+    print(f'pairs per light position {needs}', flush=True)
+    assert needs[hi] > needs[lo]
+    probe.set_light(*hi)
+    ref_hi = probe.render().clone()
+    probe.set_light(*lo)
+    probe.render()
+    ref_lo_flare = probe.flare.clone()
+
+    # capacity exactly the low light's need, never fitted: the high light
+    # overflows and clears the flare
+    pipe = FramePipeline(wl.project, wl.lens_model, wl.paths, wl.wavelengths,
+                         pair_capacity=needs[lo])
+    pipe.set_light(*lo)
+    pipe.render()
+    assert torch.equal(pipe.flare, ref_lo_flare)
+    pipe.set_light(*hi)
+    pipe.render()
+    required, overflow = pipe.raster_status()
+    assert overflow and required == needs[hi]
+    assert not pipe.flare.any()
+
+    # fitted: set_light grows the capacity and the frame is the high light's
+    pipe = FramePipeline(wl.project, wl.lens_model, wl.paths, wl.wavelengths,
+                         pair_capacity=needs[lo])
+    pipe.set_light(*lo)
+    pipe.fit_capacity()
+    pipe.set_light(*hi)
+    assert pipe.pair_capacity >= needs[hi]
+    out = pipe.render()
+    torch.cuda.synchronize()
+    assert not pipe.raster_status()[1]
+    assert torch.equal(out, ref_hi)
