@@ -321,6 +321,26 @@ int rf_flare(const float *planes, const float *ghost, const float *light_spectru
              size_t workspace_bytes, float *image, rf_stream_t stream);
 
 /*
+ * rf_flare with no workspace argument and no allocation: the workspace is a
+ * static device array of RF_FLARE_STATIC_SCRATCH_BYTES (256 MiB) that the
+ * library holds on each device it runs on until the process ends.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  The pair capacity is the largest the array holds; a
+ * frame whose workspace exceeds the array even with no pairs fails with
+ * RF_ERR_WORKSPACE before any launch.  status (device memory, may be NULL)
+ * receives {pairs the frame needed, 1 if they overflowed the capacity else
+ * 0}; on overflow the image is cleared, as with rf_flare.  Every call on one
+ * device shares that device's array, which keeps the last call's contents:
+ * calls that overlap on one device (different streams or threads) race, so
+ * serialize them.
+ */
+#define RF_FLARE_STATIC_SCRATCH_BYTES ((size_t)256 << 20)
+int rf_flare_static(const float *planes, const float *ghost, const float *light_spectrum,
+                    const rf_raster_params *prm, float *image, uint64_t *status,
+                    rf_stream_t stream);
+
+/*
  * Reads back (synchronizing `stream`) the pair count the last rf_rasterize /
  * rf_flare on this workspace needed and whether it overflowed the capacity.
  * No reference counterpart: the reference sizes its bin queues on the host

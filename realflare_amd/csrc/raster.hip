@@ -24,6 +24,8 @@
 //              registers; one coalesced write per pixel, y flipped.
 // A pixel therefore sums exactly the fragments the reference sums, in the
 // same order.
+#include <mutex>
+
 #include "primitives.h"
 
 namespace rf {
@@ -1771,3 +1773,83 @@ extern "C" int rf_raster_status(const void* workspace, uint64_t* required_pairs,
     return RF_OK;
 }
 
+// ---------------------------------------------------------------------------
+// rf_flare_static: rf_flare whose workspace is a file-scope device array, for
+// callers that cannot allocate.  The host refers to the array only through
+// the device address hipGetSymbolAddress returns, kept per device (each
+// device holds its own copy of the array).
+
+static __device__ __align__(256) unsigned char scratch[RF_FLARE_STATIC_SCRATCH_BYTES];
+
+static constexpr int RF_STATIC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mutex;
+static void* g_scratch_ptr[RF_STATIC_MAX_DEVICES];
+
+// The current device's address of `scratch`, looked up on its first use on
+// that device.
+static int static_scratch(void** out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return check_hip(e, "rf_flare_static: hipGetDevice");
+    if (dev < 0 || dev >= RF_STATIC_MAX_DEVICES)
+        return set_error(RF_ERR_INVALID, "rf_flare_static: device %d beyond the %d supported", dev,
+                         RF_STATIC_MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return check_hip(e, "rf_flare_static: hipGetSymbolAddress");
+        if (!p) return set_error(RF_ERR_HIP, "rf_flare_static: no device address for the scratch");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return RF_OK;
+}
+
+__global__ void k_flare_status(const Header* __restrict__ h, uint64_t* __restrict__ status) {
+    if (threadIdx.x == 0) {
+        status[0] = h->required;
+        status[1] = h->overflow;
+    }
+}
+
+// Largest pair capacity whose rf_flare workspace fits `bytes` (-1 if none).
+static long static_capacity(const Geom& G, size_t bytes) {
+    if (flare_layout(G, 0).total > bytes) return -1;
+    uint64_t lo = 0, hi = bytes / 8 + 1;  // a pair takes at least 8 bytes
+    while (lo < hi) {
+        const uint64_t mid = lo + (hi - lo + 1) / 2;
+        if (flare_layout(G, mid).total <= bytes) lo = mid;
+        else hi = mid - 1;
+    }
+    return (long)lo;
+}
+
+extern "C" int rf_flare_static(const float* planes, const float* ghost, const float* light_spectrum,
+                               const rf_raster_params* prm, float* image, uint64_t* status,
+                               rf_stream_t stream) {
+    Geom G;
+    int rc = make_geom(prm, &G);
+    if (rc) return rc;
+    const long cap = static_capacity(G, RF_FLARE_STATIC_SCRATCH_BYTES);
+    if (cap < 0)
+        return set_error(RF_ERR_WORKSPACE,
+                         "rf_flare_static: the frame needs %zu bytes before any binning pair, "
+                         "above the %zu-byte static scratch; use rf_flare with a workspace",
+                         flare_layout(G, 0).total, (size_t)RF_FLARE_STATIC_SCRATCH_BYTES);
+    if (!image || !ghost || (G.W > 1 && !light_spectrum) || (G.NV > 0 && !planes))
+        return set_error(RF_ERR_INVALID, "rf_flare_static: NULL buffer");
+    void* ws = nullptr;
+    rc = static_scratch(&ws);
+    if (rc) return rc;
+    rc = rf_flare(planes, ghost, light_spectrum, prm, (uint64_t)cap, ws,
+                  RF_FLARE_STATIC_SCRATCH_BYTES, image, stream);
+    if (rc) return rc;
+    if (status) {
+        // the raster header sits at the start of the flare workspace
+        hipLaunchKernelGGL(k_flare_status, dim3(1), dim3(64), 0, (hipStream_t)stream,
+                           (const Header*)ws, status);
+        RF_LAUNCH_CHECK("rf_flare_static");
+    }
+    return RF_OK;
+}

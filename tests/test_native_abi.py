@@ -59,3 +59,31 @@ def test_raster_ghost_texture_limits_without_gpu():
     rc = fn(None, None, None, ctypes.byref(prm), ctypes.c_uint64(0), None, ctypes.c_size_t(0), None)
     assert rc == native.RF_ERR_INVALID
     assert 'ghost texture side' in native.last_error()
+
+
+def test_flare_static_rejects_frames_beyond_its_scratch_without_gpu():
+    """rf_flare_static's workspace is a fixed device array: a frame whose
+    rf_flare workspace exceeds it even with no binning pairs is refused
+    before anything touches the device, with the sizes in the message."""
+    from realflare_amd import native
+    from realflare_amd.api.tasks.rasterizing import raster_params
+
+    lib = native.lib()
+    size = lib.rf_flare_workspace_size
+    size.restype = ctypes.c_size_t
+    limit = 256 << 20
+    # 66 ghosts x 8 wavelengths x 128^2 rays at 2048^2: far above 256 MiB
+    big = raster_params(66, 8, 128, 50.0, (2048, 2048), 64, 1, 1, (36.0, 24.0), 0.01, 1.0, 8.0,
+                        (256, 256), 340)
+    assert size(ctypes.byref(big), ctypes.c_uint64(0)) > limit
+    fn = lib.rf_flare_static
+    fn.restype = ctypes.c_int
+    rc = fn(None, None, None, ctypes.byref(big), None, None, None)
+    assert rc == native.RF_ERR_WORKSPACE
+    assert 'static scratch' in native.last_error() and str(limit) in native.last_error()
+    # a small frame fits (its NULL buffers are then the error)
+    small = raster_params(15, 3, 64, 50.0, (512, 512), 64, 1, 1, (36.0, 24.0), 0.01, 1.0, 8.0,
+                          (128, 128), 340)
+    assert size(ctypes.byref(small), ctypes.c_uint64(1 << 20)) < limit
+    rc = fn(None, None, None, ctypes.byref(small), None, None, None)
+    assert rc == native.RF_ERR_INVALID and 'NULL' in native.last_error()
