@@ -1,95 +1,108 @@
-"""Command-line rendering (realflare/cli/app.py:22-120, realflare/__main__.py):
-a project file, optional per-frame animation data, frames start..end, the
-output element written to the output path for every frame."""
+"""Command-line rendering, the reference's ``python -m realflare`` without the
+GUI (realflare/cli/app.py:44-120, realflare/__main__.py:10-66): a project
+file, optional animation data, a frame range, and one output element written
+for every frame.
+
+A ``RenderJob`` is built from the files first: the project with the command
+line's overrides (output path, colorspace, element; writing switched on) and
+the animation compiled into tracks (``realflare_amd.animation``).  Running it
+builds each frame's project from the tracks and renders it with one Engine;
+the Engine writes the image (engine.py:239-291).  Errors are
+``RealflareError`` with the reference's messages.
+"""
 from __future__ import annotations
 
 import argparse
-import dataclasses
 import logging
 import os
 import sys
-from typing import Any
+from dataclasses import dataclass, field
 
+from . import animation
 from .api import storage
-from .api.data import Project, RealflareError, RenderElement, cast, field_types
-from .api.engine import Engine
+from .api.data import Project, RealflareError, RenderElement, cast
 
 logger = logging.getLogger(__name__)
 
 
-def apply_animation(obj: Any, animation: dict, index: int) -> None:
-    """cli/app.py:22-41: list values are per-frame values (missing frames keep
-    the current value), dicts recurse into attributes."""
-    for name, value in animation.items():
-        if isinstance(value, list):
-            try:
-                current = value[index]
-            except IndexError:
-                continue
-            if dataclasses.is_dataclass(obj):
-                fields = field_types(type(obj))
-                if name in fields:
-                    current = cast(fields[name], current)
-            setattr(obj, name, current)
-        else:
-            try:
-                child = getattr(obj, name)
-            except AttributeError:
-                continue
-            apply_animation(child, value, index)
+def _load_json(path: str, what: str):
+    try:
+        return storage.read_data(path)
+    except ValueError as e:
+        logger.debug(e)
+        raise RealflareError(f'{what} is not valid: {path}') from None
+
+
+@dataclass
+class RenderJob:
+    project: Project
+    tracks: list = field(default_factory=list)
+    frame_start: int = 1
+    frame_end: int = 1
+
+    @classmethod
+    def from_files(cls, project_path: str, animation_path: str = '', output_path: str = '',
+                   colorspace: str = '', element: str = '', frame_start: int = 1,
+                   frame_end: int = 1) -> 'RenderJob':
+        if not project_path or not os.path.isfile(project_path):
+            raise RealflareError(f'project path not valid: {project_path}')
+        project = cast(Project, _load_json(project_path, 'project'))
+        out = project.output
+        out.write = True
+        out.path = output_path or out.path
+        out.colorspace = colorspace or out.colorspace
+        if element in RenderElement.__members__:
+            out.element = RenderElement[element]
+        tracks = []
+        if animation_path:
+            tracks = animation.compile_tracks(project, _load_json(animation_path, 'animation'))
+        return cls(project, tracks, frame_start, frame_end)
+
+    def frames(self):
+        return animation.frames(self.project, self.tracks, self.frame_start, self.frame_end)
+
+    def run(self, engine=None) -> list[int]:
+        """Render every frame; returns the frame numbers rendered."""
+        if engine is None:
+            from .api.engine import Engine
+
+            engine = Engine()
+        engine.set_elements([self.project.output.element])
+        done = []
+        for number, project in self.frames():
+            if not engine.render(project):
+                raise RealflareError('an error occurred while rendering')
+            done.append(number)
+        return done
 
 
 def render(project_path: str, animation_path: str = '', output_path: str = '',
            colorspace: str = '', element: str = '', frame_start: int = 1,
-           frame_end: int = 1) -> None:
-    """cli/app.py:44-102."""
-    if not project_path or not os.path.isfile(project_path):
-        raise RealflareError(f'project path not valid: {project_path}')
-    try:
-        data = storage.read_data(project_path)
-    except ValueError as e:
-        logger.debug(e)
-        raise RealflareError(f'project is not valid: {project_path}') from None
-    project = cast(Project, data)
-    project.output.write = True
-    if output_path:
-        project.output.path = output_path
-    if colorspace:
-        project.output.colorspace = colorspace
-    if element and element in RenderElement.__members__:
-        project.output.element = RenderElement[element]
-    animation = None
-    if animation_path:
-        try:
-            animation = storage.read_data(animation_path)
-        except ValueError as e:
-            logger.debug(e)
-            raise RealflareError(f'animation is not valid: {animation_path}') from None
-    engine = Engine()
-    engine.set_elements([project.output.element])
-    for i, frame in enumerate(range(frame_start, frame_end + 1)):
-        project.output.frame = frame
-        if animation:
-            apply_animation(project, animation, i)
-        if not engine.render(project):
-            raise RealflareError('an error occurred while rendering')
+           frame_end: int = 1) -> list[int]:
+    """cli/app.py:44-102's entry point, same arguments."""
+    job = RenderJob.from_files(project_path, animation_path, output_path, colorspace, element,
+                               frame_start, frame_end)
+    return job.run()
 
 
 def argument_parser() -> argparse.ArgumentParser:
-    """realflare/__main__.py:10-66 (without the GUI)."""
-    parser = argparse.ArgumentParser(prog='realflare_amd', description='Physically-Based Lens Flares',
-                                     formatter_class=argparse.ArgumentDefaultsHelpFormatter)
-    parser.add_argument('--project', type=str, help='the project to render, a path to a .json file')
-    parser.add_argument('--animation', type=str, help='path to a .json file containing animation data')
-    parser.add_argument('--output', type=str, help='Output image')
-    parser.add_argument('--element', type=str,
-                        help='element to render, for example: STARBURST_APERTURE, STARBURST ... '
-                             'FLARE, FLARE_STARBURST')
-    parser.add_argument('--colorspace', type=str, default='ACES - ACEScg', help='output colorspace')
-    parser.add_argument('--frame-start', type=int, default=1, help='start frame number')
-    parser.add_argument('--frame-end', type=int, default=1, help='end frame number')
-    parser.add_argument('--log', type=int, default=logging.WARNING, help='logging level')
-    return parser
+    """The options of realflare/__main__.py:10-66 that do not concern the GUI."""
+    p = argparse.ArgumentParser(prog='realflare_amd', description='Physically-Based Lens Flares',
+                                formatter_class=argparse.ArgumentDefaultsHelpFormatter)
+    options = (
+        ('--project', str, None, 'the project to render, a path to a .json file'),
+        ('--animation', str, None, 'path to a .json file containing animation data'),
+        ('--output', str, None, 'Output image'),
+        ('--element', str, None, 'element to render, for example: STARBURST_APERTURE, '
+                                 'STARBURST ... FLARE, FLARE_STARBURST'),
+        ('--colorspace', str, 'ACES - ACEScg', 'output colorspace'),
+        ('--frame-start', int, 1, 'start frame number'),
+        ('--frame-end', int, 1, 'end frame number'),
+        ('--log', int, logging.WARNING, 'logging level'),
+    )
+    for flag, tp, default, text in options:
+        p.add_argument(flag, type=tp, default=default, help=text)
+    return p
 
 
 def main(argv=None) -> int:
