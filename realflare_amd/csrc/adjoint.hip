@@ -902,6 +902,9 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_trace_fit(FitArgs a) {
             const float xt = a.tx[idx], yt = a.ty[idx], rt = a.tr[idx];
             if (isnan(xt) || isnan(yt) || isnan(rt)) continue;
             const float dx = s.pos.x - xt, dy = s.pos.y - yt, dr = (s.refl - rt) * a.inv_refl_scale;
+            // a ray reaching the sensor plane almost parallel to it lands at
+            // an overflowing position: no finite residual, no loss term
+            if (!isfinite(dx) || !isfinite(dy) || !isfinite(dr)) continue;
             float gx, gy, gr;
             const float lx = fit_term(dx, a.huber, &gx), ly = fit_term(dy, a.huber, &gy);
             const float lr = fit_term(dr, a.huber, &gr);

@@ -126,6 +126,11 @@ class FitBatch:
         self.refl_scale = 1.0
         self.loss_scale = 1.0
         self.huber_delta = 0.0  # 0: squared error; > 0: Huber beyond this residual
+        # half extents in mm of the sensor region the image shows: pixel =
+        # pos * st + res / 2 with st = width / (sensor diagonal / 2)
+        # (rasterizing.py:158-167, rasterizing.cl:368)
+        st = resolution[0] / (math.hypot(sensor[0], sensor[1]) / 2)
+        self.window = (resolution[0] / 2 / st, resolution[1] / 2 / st)
 
     def upload(self, params: list[LensParams]) -> None:
         recs = np.zeros(self.n_elements, lens_element_dtype)  # keeps the 64-byte records
@@ -150,14 +155,23 @@ class FitBatch:
                     ctypes.c_float(self.huber_delta), p(self.ws),
                     ctypes.c_size_t(self.ws.numel()), p(self.grad), p(self.fwd), native.stream())
 
-    def set_targets(self, params: list[LensParams]) -> None:
+    def set_targets(self, params: list[LensParams], image_window: bool = True) -> None:
         """Targets = the forward trace of `params` (made by the same kernel
-        with no target, so it runs no reverse pass)."""
+        with no target, so it runs no reverse pass).  With `image_window`,
+        only rays whose target lands inside the rendered image are fitted
+        (the others get no target): ghost rays that reach the sensor plane at
+        grazing angles land up to ~1e6 mm away, never show in the flare, and
+        would otherwise dominate the squared error."""
         nan = torch.full((self.rays,), float('nan'), dtype=torch.float32, device=self.device)
         self.targets = (nan, nan, nan)
         self.upload(params)
         self.launch()
         t = self.fwd.clone()
+        if image_window:
+            xw, yw = self.window
+            for i in range(0, self.rays, 1 << 26):
+                x, y, r = t[0, i:i + (1 << 26)], t[1, i:i + (1 << 26)], t[2, i:i + (1 << 26)]
+                r.masked_fill_((x.abs() > xw) | (y.abs() > yw), float('nan'))
         self.targets = (t[0], t[1], t[2])
         # RMS of the target reflectances and their count, in chunks (the
         # batch can hold billions of rays)
@@ -301,7 +315,7 @@ def fit(batch: FitBatch, start: list[LensParams], steps: int, lr: float = 1e-4,
         for k in range(n):
             if per[k] < best_l[k]:
                 best_l[k], best_p[k] = per[k], params[k]
-            elif per[k] > patience * best_l[k]:
+            elif not math.isfinite(per[k]) or per[k] > patience * best_l[k]:
                 restart.append(k)
         rec['best_loss'] = sum(best_l)
         rec['instance_loss'] = per

@@ -25,6 +25,8 @@ from helpers import trace_inputs
 from raster_helpers import (gpu_flare, gpu_trace_planes, make_ghost, oracle_image, params_for,
                             planes_to_oracle_rays, rel_l1)
 
+from realflare_amd.api.color import light_spectrum
+
 pytestmark = pytest.mark.gpu
 
 
@@ -110,6 +112,38 @@ def test_flare_end_to_end(name, g, res, oracle_lib):
     print(f'{name}: image rel L1 {err:.3e}', flush=True)
     assert np.abs(oimg).sum() > 0
     assert err <= 1e-3
+
+
+def test_image_error_against_fp64(oracle_lib):
+    """The small configuration end to end (64x64 rays, 15 ghosts, 3
+    wavelengths, 512^2): the GPU image and the FP32 oracle image, each
+    against the FP64 oracle image (FP64 trace and FP64 rasterizer).  Both
+    FP32 renders carry the same kind of error (sensor positions rounded to
+    FP32 move coverage edges), so the GPU's error is bounded by the FP32
+    oracle's own plus a margin, and by the reference's image bar."""
+    inp = trace_inputs('small')
+    print(f'case raster/small vs FP64 g={inp["grid_count"]} 512^2', flush=True)
+    ghost = make_ghost(oracle_lib)
+    prm = params_for(inp, (512, 512), ghost.shape)
+    args = (inp['lens'], inp['paths'], inp['wavelengths'], inp['aperture_index'],
+            inp['coating_min_ior'], inp['grid_count'], inp['grid_length'], inp['direction'])
+    g = inp['grid_count']
+    rays64 = oracle_lib.trace(*args, precision='f64')
+    b64, i64 = oracle_lib.prim_shader(rays64, g, prm.area_orig, prm.min_area, precision='f64')
+    v64 = oracle_lib.vertex_shader(rays64, i64, g, prm.screen_transform, (prm.width, prm.height),
+                                   precision='f64')
+    img64 = oracle_lib.rasterize(v64, b64, prm.wavelength_sub_count, g, prm.anti_aliasing,
+                                 prm.intensity, prm.ghost_scale, ghost, light_spectrum(),
+                                 prm.screen_transform, prm.bin_size, (prm.width, prm.height),
+                                 precision='f64')
+    *_, img32 = oracle_image(oracle_lib, oracle_lib.trace(*args), inp, prm, ghost)
+    img, _ = gpu_flare(gpu_trace_planes(inp), ghost, prm)
+    e_gpu, e_o32 = rel_l1(img, img64), rel_l1(img32, img64)
+    print(f'image rel L1 vs FP64: gpu {e_gpu:.3e}, fp32 oracle {e_o32:.3e}; '
+          f'gpu vs fp32 oracle {rel_l1(img, img32):.3e}', flush=True)
+    assert np.abs(img64).sum() > 0
+    assert e_gpu <= 1e-3
+    assert e_gpu <= 2 * e_o32 + 1e-6
 
 
 @pytest.mark.parametrize('aa,wsub,bin_size,wl', [(4, 3, 64, None), (2, 1, 48, None),
