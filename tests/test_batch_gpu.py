@@ -7,6 +7,8 @@
   rasterization per ghost and frame, added into the frame) within 1e-5
   relative L1: the two sum the same fragments; the plain form converts each
   ghost to AP1 separately and adds images, the batch converts per pixel once.
+* One frame of the batch equals the oracle's rasterization of the same
+  traced planes, ghost by ghost, summed (1e-5 relative L1).
 * Frames per pass and the XCD super-block size do not change the images
   (bit-exact); a captured HIP graph replays to the eager result, with eager
   work between replays.
@@ -74,6 +76,32 @@ def test_batch_images_equal_plain_per_ghost():
     for f in range(br.F):
         err = _rel_l1(batch[f], ref[f])
         print(f'frame {f}: rel L1 {err:.2e}', flush=True)
+        assert err <= 1e-5
+
+
+def test_batch_frame_matches_oracle(oracle_lib):
+    from parity import oracle_ghost_rows
+
+    wl, pos, grids = _setup()
+    br = BatchRenderer(wl.project, wl.lens_model, wl.paths, wl.wavelengths, pos, grids,
+                       Tuning(frames_per_pass=2))
+    br.fit_capacity()
+    imgs = br.render().cpu().numpy()
+    ghost = br.ghost_tex.cpu().numpy()
+    spectrum = br.spectrum.cpu().numpy()
+    planes = br.planes.cpu().numpy()
+    for f in range(br.F):
+        ref = np.zeros(imgs[f].shape)
+        for p in range(br.P):
+            s = br.seg[f * br.P + p]
+            g = int(s['grid_count'])
+            ro = int(s['ray_offset'])
+            ref += oracle_ghost_rows(oracle_lib, planes[:, ro:ro + br.W * g * g], g, br.prm, ghost,
+                                     spectrum, float(s['grid_length']), float(br.prm.min_area),
+                                     None)
+        err = _rel_l1(imgs[f], ref)
+        print(f'frame {f} vs oracle: rel L1 {err:.2e}', flush=True)
+        assert np.abs(ref).sum() > 0
         assert err <= 1e-5
 
 

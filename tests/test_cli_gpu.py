@@ -1,6 +1,7 @@
 """The command line end to end: a project file, an animation file and frames
 1..2 render the output element to EXR files equal to direct Engine renders of
-the animated projects (cli/app.py:44-102, engine.py:239-291)."""
+the animated projects (cli/app.py:44-102, engine.py:239-291); the first
+frame also matches the oracle's render of its project (1e-3 relative L1)."""
 import json
 
 import numpy as np
@@ -15,7 +16,7 @@ from realflare_amd.synthetic import workload
 pytestmark = pytest.mark.gpu
 
 
-def test_cli_renders_animated_frames(tmp_path):
+def test_cli_renders_animated_frames(tmp_path, oracle_lib):
     wl = workload('small')
     p = wl.project
     p.render.grid_count = 24
@@ -46,3 +47,12 @@ def test_cli_renders_animated_frames(tmp_path):
         for k, c in enumerate('RGB'):
             np.testing.assert_array_equal(exr[c], ref[..., k])
         assert np.abs(ref).sum() > 0
+        if i == 0:
+            from parity import oracle_flare
+
+            ghost = engine.ghost(p).array.astype(np.float32)
+            o = oracle_flare(oracle_lib, p, wl.lens_model, (x, y), ghost)
+            got = np.stack([exr[c] for c in 'RGB'], -1)
+            err = np.abs(got - o[..., :3]).sum() / np.abs(o[..., :3]).sum()
+            print(f'cli frame 1 vs oracle: rel L1 {err:.2e}', flush=True)
+            assert err <= 1e-3
