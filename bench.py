@@ -175,7 +175,7 @@ def adjoint_timing(wl, steps: int) -> dict:
     direction = light_direction((p.flare.light.position.x(), p.flare.light.position.y()), res,
                                 sensor, wl.lens_model.focal_length)
     out = {}
-    for K in (1, 4):
+    for K in (1, 2, 3, 4):
         spec = TraceSpec(static, wl.paths, wl.wavelengths, p.flare.lens.coating_min_ior, 64,
                          p.render.grid_length, direction, checkpoint_every=K)
         ones = torch.ones(spec.shape, dtype=torch.float32, device='cuda')
@@ -216,6 +216,8 @@ def adjoint_timing(wl, steps: int) -> dict:
             times[name + '_ms'] = a.elapsed_time(b) / steps
         times['backward_over_forward'] = times['backward_ms'] / times['forward_ms']
         times['fused_over_forward'] = times['fused_ms'] / times['forward_ms']
+        times['gradient_over_forward'] = ((times['forward_checkpointed_ms'] + times['backward_ms'])
+                                          / times['forward_ms'])
         out[f'K{K}'] = times
     out['rays'] = 66 * 8 * 64 * 64
     return out
