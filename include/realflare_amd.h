@@ -401,9 +401,12 @@ typedef struct rf_aperture_params {
 int rf_aperture(const rf_aperture_params *prm, float *image, rf_stream_t stream);
 
 /*
- * In-place 2-D FFT of interleaved complex64 data[height][width] (powers of
- * two up to 4096), unnormalized, then multiplied by `scale`: numpy's fft2 is
- * scale 1, ifft2 is inverse with 1/(w h), norm='ortho' is 1/sqrt(w h).
+ * In-place 2-D FFT of interleaved complex64 data[height][width], unnormalized,
+ * then multiplied by `scale`: numpy's fft2 is scale 1, ifft2 is inverse with
+ * 1/(w h), norm='ortho' is 1/sqrt(w h).  Each side is a power of two up to
+ * 4096 or any other length up to 2048 (Bluestein's chirp-z transform on a
+ * power-of-two FFT of at least twice the length); larger sides are rejected
+ * with RF_ERR_INVALID.  rf_power_spectrum and rf_ghost take the same sizes.
  * Replaces the numpy.fft calls of starburst.py:71 and frft.py:27-60.
  */
 size_t rf_fft_workspace_size(int width, int height);

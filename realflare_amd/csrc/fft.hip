@@ -1,4 +1,5 @@
-// 2-D FFT, starburst power spectrum and ghost fractional Fourier transform.
+// 2-D FFT of any size, starburst power spectrum and ghost fractional Fourier
+// transform.
 //
 // The reference runs these on the host in numpy complex128:
 //   * StarburstTask.update_fourier_spectrum (realflare/api/tasks/starburst.py:45-81):
@@ -8,7 +9,9 @@
 // Here they are complex64 on the device.  The FFT is a radix-2
 // decimation-in-time transform held entirely in LDS: a row pass (one
 // workgroup per row) and a column pass (a workgroup per group of adjacent
-// columns, so every global access reads contiguous runs of a row).  Twiddles
+// columns, so every global access reads contiguous runs of a row); a side
+// that is not a power of two goes through Bluestein's chirp-z convolution on
+// that transform, as numpy (pocketfft) transforms any length.  Twiddles
 // and chirp phases are evaluated in double and rounded once to float.
 #include <cmath>
 
@@ -101,43 +104,246 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_cols(float2* __restrict__ d
     }
 }
 
-struct FftPlan {
-    int w, h, logw, logh, cols;
-    float2 *tw_w, *tw_h;
+// A length that is not a power of two is transformed by Bluestein's
+// chirp-z identity, jk = (j^2 + k^2 - (k - j)^2) / 2:
+//   X_k = a_k sum_j (x_j a_j) b_(k-j),  a_j = exp(-i pi j^2 / n), b = conj(a),
+// the sum a circular convolution of length M = the power of two >= 2n - 1,
+// done with the radix-2 transform above: FFT, multiply by the transform of b
+// wrapped to length M (bhat), inverse FFT.  The inverse transform is the
+// conjugate of the forward transform of the conjugate.
+struct FftDim {
+    int n, M, logM;
+    bool blue;
+    float2 *tw;         // exp(-2 pi i k / M), k < M / 2
+    float2 *a, *bhat;   // Bluestein tables: a[n], bhat[M]
 };
 
+struct FftPlan {
+    FftDim x, y;
+    int cols;       // columns per workgroup of the column pass (over y.M rows)
+    float2* tmp;    // Bluestein staging: max(h * x.M, y.M * w) complex
+};
+
+static int next_pow2(int n) {
+    int m = 1;
+    while (m < n) m <<= 1;
+    return m;
+}
+
+static FftDim fft_dim(int n) {
+    FftDim d{};
+    d.n = n;
+    d.blue = !pow2(n);
+    d.M = d.blue ? next_pow2(2 * n - 1) : n;
+    d.logM = ilog2(d.M);
+    return d;
+}
+
+static size_t dim_table_bytes(const FftDim& d) {
+    size_t b = align_up((size_t)(d.M / 2 + 1) * sizeof(float2), 256);
+    if (d.blue) b += align_up((size_t)d.n * sizeof(float2), 256) +
+                     align_up((size_t)d.M * sizeof(float2), 256);
+    return b;
+}
+
+static size_t tmp_bytes(const FftDim& x, const FftDim& y) {
+    size_t n = 0;
+    if (x.blue) n = (size_t)y.n * x.M;
+    if (y.blue && (size_t)y.M * x.n > n) n = (size_t)y.M * x.n;
+    return align_up(n * sizeof(float2), 256);
+}
+
 size_t fft_workspace_bytes(int w, int h) {
-    return align_up((size_t)(w / 2 + 1) * sizeof(float2), 256) +
-           align_up((size_t)(h / 2 + 1) * sizeof(float2), 256);
+    if (w < 1 || h < 1) return 0;
+    const FftDim x = fft_dim(w), y = fft_dim(h);
+    return dim_table_bytes(x) + dim_table_bytes(y) + tmp_bytes(x, y);
+}
+
+// a[j] = exp(-i pi j^2 / n) (j^2 reduced mod 2n, exactly, before the sine)
+__global__ void k_blue_chirp(float2* __restrict__ a, int n) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const long r = ((long)j * j) % (2L * n);
+    double sn, cs;
+    sincospi(-(double)r / (double)n, &sn, &cs);
+    a[j] = make_float2((float)cs, (float)sn);
+}
+
+// b = conj(a) wrapped to length M: b[m] = conj(a[m]), b[M - m] = conj(a[m]),
+// zero between
+__global__ void k_blue_kernel(const float2* __restrict__ a, int n, int M, float2* __restrict__ b) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= M) return;
+    float2 v = make_float2(0.0f, 0.0f);
+    if (m < n) v = make_float2(a[m].x, -a[m].y);
+    else if (M - m < n) v = make_float2(a[M - m].x, -a[M - m].y);
+    b[m] = v;
+}
+
+// Rows (axis x): tmp[y][m] = (conj?)(data[y][m]) a[m] for m < n, 0 up to M.
+__global__ __launch_bounds__(256) void k_blue_pre_rows(const float2* __restrict__ data, int n,
+                                                       int rows, int M, const float2* __restrict__ a,
+                                                       int inverse, float2* __restrict__ tmp) {
+#pragma clang fp contract(off)
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long)rows * M) return;
+    const int m = (int)(i % M);
+    const long r = i / M;
+    float2 v = make_float2(0.0f, 0.0f);
+    if (m < n) {
+        float2 x = data[r * n + m];
+        if (inverse) x.y = -x.y;
+        v = cmul(x, a[m]);
+    }
+    tmp[i] = v;
+}
+
+// Columns (axis y): tmp[m][c] = (conj?)(data[m][c]) a[m] for m < n, 0 up to M.
+__global__ __launch_bounds__(256) void k_blue_pre_cols(const float2* __restrict__ data, int w, int n,
+                                                       int M, const float2* __restrict__ a,
+                                                       int inverse, float2* __restrict__ tmp) {
+#pragma clang fp contract(off)
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long)M * w) return;
+    const int m = (int)(i / w);
+    float2 v = make_float2(0.0f, 0.0f);
+    if (m < n) {
+        float2 x = data[i];
+        if (inverse) x.y = -x.y;
+        v = cmul(x, a[m]);
+    }
+    tmp[i] = v;
+}
+
+// tmp *= bhat along the transformed axis (rows: index i % M; columns: i / w)
+__global__ __launch_bounds__(256) void k_blue_mul(float2* __restrict__ tmp, long total, int M, int w,
+                                                  int along_rows, const float2* __restrict__ bhat) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= total) return;
+    const int m = along_rows ? (int)(i % M) : (int)(i / w);
+    tmp[i] = cmul(tmp[i], bhat[m]);
+}
+
+// data[y][k] = (conj?)(a[k] tmp[y][k]) * scale, k < n (tmp holds M times the
+// circular convolution: the inverse transform is unnormalized)
+__global__ __launch_bounds__(256) void k_blue_post_rows(const float2* __restrict__ tmp, int n, int rows,
+                                                        int M, const float2* __restrict__ a,
+                                                        int inverse, float scale,
+                                                        float2* __restrict__ data) {
+#pragma clang fp contract(off)
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long)rows * n) return;
+    const int k = (int)(i % n);
+    const long r = i / n;
+    float2 v = cmul(a[k], tmp[r * M + k]);
+    if (inverse) v.y = -v.y;
+    data[i] = make_float2(v.x * scale, v.y * scale);
+}
+
+__global__ __launch_bounds__(256) void k_blue_post_cols(const float2* __restrict__ tmp, int w, int n,
+                                                        const float2* __restrict__ a, int inverse,
+                                                        float scale, float2* __restrict__ data) {
+#pragma clang fp contract(off)
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (long)n * w) return;
+    const int k = (int)(i / w);
+    float2 v = cmul(a[k], tmp[i]);
+    if (inverse) v.y = -v.y;
+    data[i] = make_float2(v.x * scale, v.y * scale);
+}
+
+static inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
+
+static int cols_for(int M, int w) {
+    int cols = FFT_LDS_BYTES / (int)(M * sizeof(float2));
+    if (cols > 16) cols = 16;
+    if (cols > w) cols = w;
+    while (cols > 1 && w % cols) cols--;  // whole groups of columns
+    return cols < 1 ? 1 : cols;
+}
+
+static void run_rows_pow2(const FftDim& d, float2* data, int rows, bool inverse, float scale,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(k_fft_rows, dim3(rows), dim3(FFT_THREADS), d.M * sizeof(float2), s, data, d.M,
+                       d.logM, d.tw, (int)inverse, scale);
+}
+
+static void run_cols_pow2(const FftDim& d, float2* data, int w, bool inverse, float scale,
+                          hipStream_t s) {
+    const int cols = cols_for(d.M, w);
+    hipLaunchKernelGGL(k_fft_cols, dim3(w / cols), dim3(FFT_THREADS), cols * d.M * sizeof(float2), s,
+                       data, w, d.M, d.logM, cols, d.tw, (int)inverse, scale);
+}
+
+static int carve_dim(FftDim* d, char** w, hipStream_t s) {
+    d->tw = (float2*)*w;
+    *w += align_up((size_t)(d->M / 2 + 1) * sizeof(float2), 256);
+    if (d->M >= 2)
+        hipLaunchKernelGGL(k_twiddles, dim3((d->M / 2 + 255) / 256), dim3(256), 0, s, d->tw, d->M);
+    d->a = d->bhat = nullptr;
+    if (d->blue) {
+        d->a = (float2*)*w;
+        *w += align_up((size_t)d->n * sizeof(float2), 256);
+        d->bhat = (float2*)*w;
+        *w += align_up((size_t)d->M * sizeof(float2), 256);
+        hipLaunchKernelGGL(k_blue_chirp, dim3(nblk(d->n)), dim3(256), 0, s, d->a, d->n);
+        hipLaunchKernelGGL(k_blue_kernel, dim3(nblk(d->M)), dim3(256), 0, s, d->a, d->n, d->M,
+                           d->bhat);
+        run_rows_pow2(*d, d->bhat, 1, false, 1.0f, s);  // bhat = FFT_M(b)
+    }
+    return RF_OK;
 }
 
 static int fft_plan(int w, int h, void* ws, hipStream_t s, FftPlan* p) {
-    if (!pow2(w) || !pow2(h) || w > FFT_MAX_N || h > FFT_MAX_N)
-        return set_error(RF_ERR_INVALID, "fft: sizes must be powers of two <= %d (got %dx%d)",
-                         FFT_MAX_N, w, h);
-    p->w = w;
-    p->h = h;
-    p->logw = ilog2(w);
-    p->logh = ilog2(h);
-    int cols = FFT_LDS_BYTES / (int)(h * sizeof(float2));
-    if (cols > 16) cols = 16;
-    if (cols > w) cols = w;
-    p->cols = cols < 1 ? 1 : cols;
-    p->tw_w = (float2*)ws;
-    p->tw_h = (float2*)((char*)ws + align_up((size_t)(w / 2 + 1) * sizeof(float2), 256));
-    if (w >= 2) hipLaunchKernelGGL(k_twiddles, dim3((w / 2 + 255) / 256), dim3(256), 0, s, p->tw_w, w);
-    if (h >= 2) hipLaunchKernelGGL(k_twiddles, dim3((h / 2 + 255) / 256), dim3(256), 0, s, p->tw_h, h);
-    RF_LAUNCH_CHECK("fft twiddles");
+    if (w < 1 || h < 1)
+        return set_error(RF_ERR_INVALID, "fft: empty size %dx%d", w, h);
+    p->x = fft_dim(w);
+    p->y = fft_dim(h);
+    if (p->x.M > FFT_MAX_N || p->y.M > FFT_MAX_N)
+        return set_error(RF_ERR_INVALID,
+                         "fft: a side must be a power of two up to %d or any length up to %d "
+                         "(got %dx%d)",
+                         FFT_MAX_N, FFT_MAX_N / 2, w, h);
+    char* c = (char*)ws;
+    carve_dim(&p->x, &c, s);
+    carve_dim(&p->y, &c, s);
+    p->tmp = (float2*)c;
+    p->cols = cols_for(p->y.M, w);
+    RF_LAUNCH_CHECK("fft plan");
     return RF_OK;
 }
 
 // unnormalized transform times `scale` (numpy: forward scale 1, inverse 1/N)
 static int fft_run(const FftPlan& p, float2* data, bool inverse, float scale, hipStream_t s) {
-    hipLaunchKernelGGL(k_fft_rows, dim3(p.h), dim3(FFT_THREADS), p.w * sizeof(float2), s, data, p.w,
-                       p.logw, p.tw_w, (int)inverse, scale);
-    hipLaunchKernelGGL(k_fft_cols, dim3(p.w / p.cols), dim3(FFT_THREADS),
-                       p.cols * p.h * sizeof(float2), s, data, p.w, p.h, p.logh, p.cols, p.tw_h,
-                       (int)inverse, 1.0f);
+    const int w = p.x.n, h = p.y.n;
+    if (!p.x.blue) {
+        run_rows_pow2(p.x, data, h, inverse, scale, s);
+    } else {
+        const FftDim& d = p.x;
+        const long total = (long)h * d.M;
+        hipLaunchKernelGGL(k_blue_pre_rows, dim3(nblk(total)), dim3(256), 0, s, data, w, h, d.M, d.a,
+                           (int)inverse, p.tmp);
+        run_rows_pow2(d, p.tmp, h, false, 1.0f, s);
+        hipLaunchKernelGGL(k_blue_mul, dim3(nblk(total)), dim3(256), 0, s, p.tmp, total, d.M, 0, 1,
+                           d.bhat);
+        run_rows_pow2(d, p.tmp, h, true, 1.0f / (float)d.M, s);
+        hipLaunchKernelGGL(k_blue_post_rows, dim3(nblk((long)h * w)), dim3(256), 0, s, p.tmp, w, h,
+                           d.M, d.a, (int)inverse, scale, data);
+    }
+    if (!p.y.blue) {
+        run_cols_pow2(p.y, data, w, inverse, 1.0f, s);
+    } else {
+        const FftDim& d = p.y;
+        const long total = (long)d.M * w;
+        hipLaunchKernelGGL(k_blue_pre_cols, dim3(nblk(total)), dim3(256), 0, s, data, w, h, d.M, d.a,
+                           (int)inverse, p.tmp);
+        run_cols_pow2(d, p.tmp, w, false, 1.0f, s);
+        hipLaunchKernelGGL(k_blue_mul, dim3(nblk(total)), dim3(256), 0, s, p.tmp, total, d.M, w, 0,
+                           d.bhat);
+        run_cols_pow2(d, p.tmp, w, true, 1.0f / (float)d.M, s);
+        hipLaunchKernelGGL(k_blue_post_cols, dim3(nblk((long)h * w)), dim3(256), 0, s, p.tmp, w, h,
+                           d.a, (int)inverse, 1.0f, data);
+    }
     RF_LAUNCH_CHECK("fft");
     return RF_OK;
 }
@@ -161,13 +367,14 @@ __global__ __launch_bounds__(256) void k_starburst_field(const float* __restrict
     out[i] = make_float2((float)(a * cs), (float)(a * sn));
 }
 
-// out = |fftshift(F)|^2 (squares summed in double, then rounded to float)
+// out = |fftshift(F)|^2 (squares summed in double, then rounded to float);
+// fftshift moves element (i - n // 2) mod n to i, for odd n as well
 __global__ __launch_bounds__(256) void k_power_shift(const float2* __restrict__ F, int w, int h,
                                                      float* __restrict__ out) {
     const long i = (long)blockIdx.x * 256 + threadIdx.x;
     if (i >= (long)w * h) return;
     const int x = (int)(i % w), y = (int)(i / w);
-    const int sx = (x + w / 2) % w, sy = (y + h / 2) % h;
+    const int sx = (x + (w + 1) / 2) % w, sy = (y + (h + 1) / 2) % h;
     const float2 v = F[(long)sy * w + sx];
     out[i] = (float)((double)v.x * v.x + (double)v.y * v.y);
 }
@@ -181,7 +388,7 @@ __global__ __launch_bounds__(256) void k_shift_real(const float* __restrict__ in
     const long i = (long)blockIdx.x * 256 + threadIdx.x;
     if (i >= (long)w * h) return;
     const int x = (int)(i % w), y = (int)(i / w);
-    const int sx = (x + w / 2) % w, sy = (y + h / 2) % h;
+    const int sx = (x + (w + 1) / 2) % w, sy = (y + (h + 1) / 2) % h;
     out[i] = make_float2(in[(long)sy * w + sx], 0.0f);
 }
 
@@ -193,11 +400,13 @@ __global__ __launch_bounds__(256) void k_flip(const float2* __restrict__ in, int
     out[i] = in[(long)(h - 1 - y) * w + (w - 1 - x)];
 }
 
-// chirp phase of frft.py:8-17 at (x, y): pi * (ky^2 / h + kx^2 / w) with
-// k the FFT-order frequency index
+// chirp phase of frft.py:8-17 at (x, y): pi * (ky^2 / h + kx^2 / w) with k
+// the element of fftshift(arange(-n // 2, n // 2)) (floor division), which is
+// x for x < n // 2 and x - n from there (for odd n the grid runs from
+// -(n + 1) / 2, so it is not the FFT-order frequency index)
 __device__ __forceinline__ double chirp_phase(int x, int y, int w, int h) {
-    const double kx = x < (w + 1) / 2 ? x : x - w;
-    const double ky = y < (h + 1) / 2 ? y : y - h;
+    const double kx = x < w / 2 ? x : x - w;
+    const double ky = y < h / 2 ? y : y - h;
     return 3.14159265358979323846 * (ky * ky / h + kx * kx / w);
 }
 
@@ -229,7 +438,7 @@ __global__ __launch_bounds__(256) void k_frft_post(const float2* __restrict__ y,
     const long i = (long)blockIdx.x * 256 + threadIdx.x;
     if (i >= (long)w * h) return;
     const int x = (int)(i % w), yy = (int)(i / w);
-    const int sx = (x + w / 2) % w, sy = (yy + h / 2) % h;
+    const int sx = (x + (w + 1) / 2) % w, sy = (yy + (h + 1) / 2) % h;
     const long j = (long)sy * w + sx;
     double s1, k1;
     sincos(chirp_phase(sx, sy, w, h) * c1, &s1, &k1);
@@ -238,8 +447,6 @@ __global__ __launch_bounds__(256) void k_frft_post(const float2* __restrict__ y,
     const double re = ar * vr - ai * vi, im = ar * vi + ai * vr;
     out[i] = (float)sqrt(re * re + im * im) * (float)amp;
 }
-
-static inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace rf
 

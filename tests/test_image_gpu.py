@@ -89,7 +89,10 @@ def test_fft2_matches_numpy():
     from realflare_amd import native
 
     rng = np.random.default_rng(1)
-    for h, w in [(128, 256), (64, 64), (1024, 32)]:
+    # powers of two, and sides of any length (Bluestein), odd, prime, mixed,
+    # and the largest of each kind
+    for h, w in [(128, 256), (64, 64), (1024, 32), (1000, 600), (37, 12), (5, 1), (97, 128),
+                 (2048, 3), (6, 4096)]:
         x = (rng.standard_normal((h, w)) + 1j * rng.standard_normal((h, w))).astype(np.complex64)
         for inverse in (0, 1):
             t = torch.from_numpy(x.view(np.float32).copy()).cuda()
@@ -105,6 +108,52 @@ def test_fft2_matches_numpy():
             err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
             print(f'fft {h}x{w} inverse={inverse} rel err {err:.2e}', flush=True)
             assert err <= 1e-5
+
+
+def test_fft_size_limits():
+    from realflare_amd import native
+    from realflare_amd.api.data import RealflareError
+
+    x = torch.zeros((2 * 2049,), dtype=torch.float32, device='cuda')
+    ws = torch.empty((1 << 20,), dtype=torch.uint8, device='cuda')
+    for w, h in ((2049, 1), (1, 8192)):
+        with pytest.raises(RealflareError, match='power of two up to 4096'):
+            native.call('rf_fft2', native.ptr(x), w, h, 0, ctypes.c_float(1.0), native.ptr(ws),
+                        ctypes.c_size_t(ws.numel()), native.stream())
+
+
+@pytest.mark.parametrize('size', [(1000, 1000), (300, 200), (301, 199)])
+def test_power_spectrum_and_ghost_any_size(size, oracle_lib):
+    """Sizes a project may hold that are not powers of two (a 1000x1000
+    starburst or ghost): the same bars as the power-of-two sizes."""
+    from realflare_amd import native
+    from realflare_amd.api.data import Aperture
+
+    w, h = size
+    a = Aperture()
+    a.shape.blades, a.shape.softness = 7, 0.1
+    ap, _ = _gpu_aperture(a, (w, h))
+    ap_t = torch.from_numpy(ap).cuda()
+    fn = native.lib().rf_power_spectrum_workspace_size
+    fn.restype = ctypes.c_size_t
+    ws = torch.empty((fn(w, h),), dtype=torch.uint8, device='cuda')
+    out = torch.empty((h, w), dtype=torch.float32, device='cuda')
+    native.call('rf_power_spectrum', native.ptr(ap_t), w, h, ctypes.c_float(1.0), native.ptr(ws),
+                ctypes.c_size_t(ws.numel()), native.ptr(out), native.stream())
+    ref = oracle_lib.power_spectrum(ap.astype(np.float64), 1.0)
+    err = rel_l1(out.cpu().numpy(), ref)
+    print(f'power spectrum {w}x{h} rel L1 {err:.2e}', flush=True)
+    assert err <= 1e-4
+    fn = native.lib().rf_ghost_workspace_size
+    fn.restype = ctypes.c_size_t
+    ws = torch.empty((fn(w, h),), dtype=torch.uint8, device='cuda')
+    for fstop in (2.0, 16.0, 120.0):
+        native.call('rf_ghost', native.ptr(ap_t), w, h, ctypes.c_float(fstop), native.ptr(ws),
+                    ctypes.c_size_t(ws.numel()), native.ptr(out), native.stream())
+        ref = oracle_lib.ghost(ap, fstop, (w, h))
+        err = rel_l1(out.cpu().numpy(), ref)
+        print(f'ghost {w}x{h} fstop={fstop} rel L1 {err:.2e}', flush=True)
+        assert err <= 1e-4
 
 
 def test_power_spectrum_and_ghost_match_oracle(oracle_lib):
