@@ -384,6 +384,68 @@ def fit_timing(timed_steps: int = 3, fit_steps: int = 100) -> dict:
     }
 
 
+def reference_benchmark(frames: int = 3) -> dict:
+    """The reference's own benchmark (benchmark/benchmark.py with
+    benchmark/nikon_ai_50_135mm): the Nikon AI 50-135mm project (1920x1080,
+    anti-aliasing 4, 5 wavelengths x 8 sub-steps, 129^2 rays, cull 0.5, ghost
+    512^2, starburst 512^2 x 1000 samples), FLARE_STARBURST rendered by
+    Engine.render for the animation's light positions (0.6, 0.3), (0.3, 0.6),
+    the last held for the frames after it.  The stage times are parsed from
+    the timer log lines as that script parses them (device-synchronized
+    stages); the first frame (workspace sizing) is reported apart from the
+    mean of the others."""
+    import logging
+    import re
+    import statistics
+
+    from realflare_amd.api.data import Point, Project, RenderElement, cast
+    from realflare_amd.api.engine import Engine
+
+    here = os.path.dirname(os.path.abspath(__file__))
+    with open(os.path.join(here, 'tests', 'golden', 'nikon_benchmark_project.json')) as f:
+        project = cast(Project, json.load(f))
+    project.output.write = False
+    positions = [(0.6, 0.3), (0.3, 0.6)]
+    lines = []
+
+    class Grab(logging.Handler):
+        def emit(self, record):
+            lines.append(record.getMessage())
+
+    log = logging.getLogger('realflare_amd.utils.timing')
+    handler = Grab(level=logging.INFO)
+    old_level = log.level
+    log.setLevel(logging.INFO)
+    log.addHandler(handler)
+    pattern = re.compile(r'([\w.]+):\s*(\d+(?:\.\d+)?)ms$')
+    per_frame = []
+    try:
+        engine = Engine()
+        engine.set_elements([RenderElement.FLARE_STARBURST])
+        for i in range(frames):
+            project.flare.light.position = Point(*positions[min(i, len(positions) - 1)])
+            lines.clear()
+            assert engine.render(project)
+            times: dict = {}
+            for line in lines:
+                m = pattern.search(line.strip())
+                if m:
+                    times.setdefault(m.group(1), []).append(float(m.group(2)))
+            per_frame.append({k: sum(v) for k, v in times.items()})
+    finally:
+        log.removeHandler(handler)
+        log.setLevel(old_level)
+    warm = per_frame[1:] or per_frame
+    keys = list(per_frame[0])
+    return {
+        'project': 'benchmark/nikon_ai_50_135mm (1920x1080, AA 4, 5 x 8 wavelength steps, '
+                   '129^2 rays, cull 0.5, starburst 512^2 x 1000 samples)',
+        'frames': frames,
+        'stage_ms_mean_after_first': {k: statistics.mean(f.get(k, 0.0) for f in warm) for k in keys},
+        'stage_ms_first_frame': per_frame[0],
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -396,6 +458,10 @@ def main() -> int:
     ap.add_argument('--no-batched', action='store_true', help='skip the batched configuration')
     ap.add_argument('--batch-frames', type=int, default=64)
     ap.add_argument('--no-fit', action='store_true', help='skip the batched fit configuration')
+    ap.add_argument('--no-reference-benchmark', action='store_true',
+                    help="skip the reference's own benchmark project")
+    ap.add_argument('--reference-benchmark-only', action='store_true',
+                    help="run only the reference's own benchmark project and print it")
     ap.add_argument('--copy-calibration', action='store_true',
                     help='also run a device-to-device copy of 256 MiB (for counter scaling)')
     args = ap.parse_args()
@@ -410,6 +476,9 @@ def main() -> int:
     from realflare_amd.synthetic import workload
 
     assert torch.cuda.is_available(), 'bench.py needs a HIP device'
+    if args.reference_benchmark_only:
+        print(json.dumps({'reference_benchmark': reference_benchmark()}), flush=True)
+        return 0
     wl = workload(args.workload)
     pipe = FramePipeline(wl.project, wl.lens_model, wl.paths, wl.wavelengths)
     print(f'# workload {args.workload}: {pipe.P} ghosts x {pipe.W} wavelengths x {pipe.g}^2 rays '
@@ -517,6 +586,9 @@ def main() -> int:
     if args.workload == 'full' and not args.no_fit:
         line['batched_fit'] = fit_timing()
         print(f"# batched fit: {line['batched_fit']}", file=sys.stderr, flush=True)
+    if args.workload == 'full' and not args.no_reference_benchmark:
+        line['reference_benchmark'] = reference_benchmark()
+        print(f"# reference benchmark: {line['reference_benchmark']}", file=sys.stderr, flush=True)
     if args.copy_calibration:
         src = torch.empty((64 << 20,), dtype=torch.float32, device='cuda')
         dst = torch.empty_like(src)

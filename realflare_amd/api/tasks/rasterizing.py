@@ -4,8 +4,9 @@ Mirrors ``RasterizingTask`` (``realflare/api/tasks/rasterizing.py:61-439``):
 the same derived settings (area_orig, min_area, screen transform, bin size,
 sub-wavelengths, ghost scale, intensity * 1e3) and ``run(project, rays,
 ghost) -> Image``; prim_shader, vertex_shader, binner and rasterizer run as
-the HIP pipeline behind ``rf_flare``.  The tile workspace is kept between
-frames and grown when a frame reports that it needs more binning pairs.
+four HIP stages (timed under the reference's names); ``flare`` runs them as
+one ``rf_flare`` call.  The tile workspace is kept between frames and grown
+when a frame reports that it needs more binning pairs.
 """
 from __future__ import annotations
 
@@ -128,16 +129,83 @@ def flare(planes: torch.Tensor, ghost: torch.Tensor, spectrum: torch.Tensor, prm
     raise RealflareError('rasterizer: binning pairs keep overflowing the workspace')
 
 
+class StageBuffers:
+    """Device buffers of the four rasterizer stages, grown on demand: bounds
+    float[P][Q][4], intensities float[P][Q][W], vertexes float[P][R][W][8]
+    and the binner / rasterizer workspace for `capacity` pairs."""
+
+    def __init__(self, dev) -> None:
+        self.device = dev
+        self.capacity = 1 << 22
+        self.bufs: dict = {}
+
+    def get(self, name: str, numel: int, dtype=torch.float32) -> torch.Tensor:
+        t = self.bufs.get(name)
+        if t is None or t.numel() < numel:
+            self.bufs[name] = None
+            t = self.bufs[name] = torch.empty((numel,), dtype=dtype, device=self.device)
+        return t
+
+    def workspace(self, prm: RasterParams) -> torch.Tensor:
+        size = native.lib().rf_rasterize_workspace_size
+        size.restype = ctypes.c_size_t
+        n = size(ctypes.byref(prm), ctypes.c_uint64(self.capacity))
+        if n == 0:
+            native.call('rf_raster_bin', None, None, None, ctypes.byref(prm), ctypes.c_uint64(0),
+                        None, ctypes.c_size_t(0), None)
+        return self.get('workspace', n, torch.uint8)
+
+
 class RasterizingTask:
+    """The four kernels of RasterizingTask.rasterize (rasterizing.py:296-421)
+    as four timed stages with the reference's names; the binner is run again
+    with a larger pair capacity when a frame needs more pairs than the
+    workspace holds, so the rasterizer never draws an overflowed binning."""
+
     def __init__(self) -> None:
         self.device = device()
         self.workspace = RasterWorkspace(self.device)
+        self.stages = StageBuffers(self.device)
         self._spectrum = None
 
     def spectrum(self) -> torch.Tensor:
         if self._spectrum is None:
             self._spectrum = torch.from_numpy(light_spectrum()).to(self.device)
         return self._spectrum
+
+    @timer
+    def prim_shader(self, planes, prm, bounds, intens) -> None:
+        native.call('rf_prim_shader', native.ptr(planes), ctypes.byref(prm), native.ptr(bounds),
+                    native.ptr(intens), native.stream())
+
+    @timer
+    def vertex_shader(self, planes, prm, intens, vertexes) -> None:
+        native.call('rf_vertex_shader', native.ptr(planes), native.ptr(intens), ctypes.byref(prm),
+                    native.ptr(vertexes), native.stream())
+
+    @timer
+    def binner(self, prm, vertexes, bounds) -> tuple[int, bool]:
+        for _ in range(3):
+            ws = self.stages.workspace(prm)
+            native.call('rf_raster_bin', native.ptr(vertexes), native.ptr(bounds),
+                        native.ptr(self.spectrum()), ctypes.byref(prm),
+                        ctypes.c_uint64(self.stages.capacity), native.ptr(ws),
+                        ctypes.c_size_t(ws.numel()), native.stream())
+            req = ctypes.c_uint64(0)
+            ovf = ctypes.c_int(0)
+            native.call('rf_raster_status', native.ptr(ws), ctypes.byref(req), ctypes.byref(ovf),
+                        native.stream())
+            if not ovf.value:
+                return int(req.value), False
+            self.stages.capacity = int(req.value * 1.25) + 1024
+        raise RealflareError('rasterizer: binning pairs keep overflowing the workspace')
+
+    @timer
+    def rasterizer(self, prm, vertexes, bounds, ghost, image) -> None:
+        ws = self.stages.workspace(prm)
+        native.call('rf_raster_draw', native.ptr(vertexes), native.ptr(bounds), native.ptr(ghost),
+                    ctypes.byref(prm), ctypes.c_uint64(self.stages.capacity), native.ptr(ws),
+                    ctypes.c_size_t(ws.numel()), native.ptr(image), native.stream())
 
     def rasterize(self, render, rays, ghost: Image, sensor_size, min_area, intensity,
                   fstop) -> Image:
@@ -152,7 +220,16 @@ class RasterizingTask:
                             render.anti_aliasing, render.wavelength_sub_count, sensor_size,
                             min_area, intensity, fstop, tuple(ghost_t.shape[:2]),
                             self.spectrum().shape[0])
-        image = flare(rays.planes, ghost_t, self.spectrum(), prm, self.workspace)
+        Q = (render.grid_count - 1) ** 2
+        st = self.stages
+        bounds = st.get('bounds', P * Q * 4)
+        intens = st.get('intens', P * Q * W)
+        vertexes = st.get('vertexes', P * R * W * 8)
+        image = torch.empty((h, w, 4), dtype=torch.float32, device=self.device)
+        self.prim_shader(rays.planes, prm, bounds, intens)
+        self.vertex_shader(rays.planes, prm, intens, vertexes)
+        self.binner(prm, vertexes, bounds)
+        self.rasterizer(prm, vertexes, bounds, ghost_t, image)
         return Image(tensor=image, args=(hash(rays), hash(ghost), w, h))
 
     @timer
