@@ -1,8 +1,11 @@
-"""Extract the data tables the render path needs from a realflare checkout.
+"""Extract the data tables the render path needs from a realflare checkout
+into ``realflare_amd/resources``.
 
-This is a one-off build step, not part of the render path: it reads data
-files (never code) from a realflare source tree and writes compact files
-into this package, so the package carries no dependency on that tree.
+A one-off step that made those data files, kept beside the other scripts
+that write data from the reference: it reads data files (never code) from a
+realflare source tree and writes compact files into the package, so the
+package carries no dependency on that tree.  tests/test_oracle_pins.py
+checks the CIE table against the reference's own rows.
 
 * ``glass/schott/catalog.json``: name, n_d, V_d and the six "formula 2"
   Sellmeier coefficients of every SCHOTT glass in
@@ -16,7 +19,7 @@ into this package, so the package carries no dependency on that tree.
 * ``model/<maker>/<name>.json``: the lens prescriptions shipped in
   ``realflare/resources/model``.
 
-Usage: ``python extract_resources.py /root/reference``
+Usage: ``python tests/golden/extract_resources.py /root/reference``
 """
 from __future__ import annotations
 
@@ -29,7 +32,8 @@ import sys
 import yaml
 
 
-HERE = os.path.dirname(os.path.abspath(__file__))
+HERE = os.path.join(os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)))),
+                    'realflare_amd', 'resources')
 
 
 def extract_glasses(src: str) -> list[dict]:
