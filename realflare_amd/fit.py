@@ -131,6 +131,16 @@ class FitBatch:
         # (rasterizing.py:158-167, rasterizing.cl:368)
         st = resolution[0] / (math.hypot(sensor[0], sensor[1]) / 2)
         self.window = (resolution[0] / 2 / st, resolution[1] / 2 / st)
+        # a ray that reaches the sensor in the target but not in the current
+        # trace adds this (times loss_scale) to the loss, with no gradient:
+        # the squared image diagonal, as far as a ray in the image can be
+        # from its target.  Without it, parameters that lose every ray would
+        # score a loss of 0.
+        self.lost_penalty = 4.0 * (self.window[0] ** 2 + self.window[1] ** 2)
+        first_ray = [int(seg[seg_off[k]]['ray_offset']) if seg_off[k] < len(seg) else self.rays
+                     for k in range(self.I)]
+        self.instance_rays = [(first_ray[k], first_ray[k + 1] if k + 1 < self.I else self.rays)
+                              for k in range(self.I)]
 
     def upload(self, params: list[LensParams]) -> None:
         recs = np.zeros(self.n_elements, lens_element_dtype)  # keeps the 64-byte records
@@ -199,11 +209,28 @@ class FitBatch:
         v = torch.cat(vals)
         return float(torch.quantile(v.float(), q)) if v.numel() else 0.0
 
+    def lost_rays(self) -> list[int]:
+        """Per instance: rays with a target that the last launch's trace lost
+        (did not reach the sensor, or reached it at a non-finite position)."""
+        tr = self.targets[2]
+        out = []
+        for lo, hi in self.instance_rays:
+            n = 0
+            for i in range(lo, hi, 1 << 26):
+                j = min(hi, i + (1 << 26))
+                f = self.fwd[:, i:j]
+                gone = torch.isnan(f[2]) | ~torch.isfinite(f[0]) | ~torch.isfinite(f[1])
+                n += int((~torch.isnan(tr[i:j]) & gone).sum())
+            out.append(n)
+        return out
+
     def step(self, params: list[LensParams]):
-        """Loss per instance and parameter gradients per instance."""
+        """Loss per instance (with the lost-ray penalty) and parameter
+        gradients per instance."""
         self.upload(params)
         self.launch()
         g = self.grad.double().cpu().numpy()
+        lost = self.lost_rays() if self.lost_penalty > 0 else [0] * self.I
         losses, grads = [], []
         LF, W = self.LF, self.W
         for k, (it, p) in enumerate(zip(self.inst, params)):
@@ -211,7 +238,7 @@ class FitBatch:
             row = g[k]
             compact = np.concatenate([row[b * LF:b * LF + L] for b in range(3 + W)])
             grads.append(chain_grads(it.static, self.wavelengths, p, compact))
-            losses.append(row[(3 + W) * LF])
+            losses.append(row[(3 + W) * LF] + self.loss_scale * self.lost_penalty * lost[k])
         return np.array(losses), grads
 
 
@@ -241,8 +268,13 @@ class Adam:
         self.t[k] = 0
 
     def update(self, params: list[LensParams], grads) -> list[LensParams]:
+        """One step per instance; an instance whose gradient is not finite
+        keeps its parameters and moments."""
         out = []
         for k, (p, g) in enumerate(zip(params, grads)):
+            if not all(bool(torch.isfinite(x).all()) for x in g):
+                out.append(p)
+                continue
             self.t[k] += 1
             t = self.t[k]
             gf = torch.cat([x.reshape(-1) for x in g]).double().numpy() * self.scale[k]
@@ -313,9 +345,10 @@ def fit(batch: FitBatch, start: list[LensParams], steps: int, lr: float = 1e-4,
             rec.update(on_step(s, params) or {})
         restart = []
         for k in range(n):
-            if per[k] < best_l[k]:
+            finite = all(bool(torch.isfinite(t).all()) for t in grads[k])
+            if finite and per[k] < best_l[k]:
                 best_l[k], best_p[k] = per[k], params[k]
-            elif not math.isfinite(per[k]) or per[k] > patience * best_l[k]:
+            elif not finite or not math.isfinite(per[k]) or per[k] > patience * best_l[k]:
                 restart.append(k)
         rec['best_loss'] = sum(best_l)
         rec['instance_loss'] = per
