@@ -145,7 +145,7 @@ class TraceSpec:
     """Non-differentiable inputs of one traced batch, resident on the device."""
 
     def __init__(self, static: LensStatic, paths, wavelengths, coating_min_ior: float,
-                 grid_count: int, grid_length: float, direction, checkpoint_every: int = 4,
+                 grid_count: int, grid_length: float, direction, checkpoint_every: int = 2,
                  device=None) -> None:
         self.static = static
         self.paths = [tuple(p) for p in paths]

@@ -16,7 +16,9 @@
 // each segment's per-event states from its checkpoint, then for every event
 // recomputes the event's intermediates (intersection, normal, Fresnel terms)
 // and applies their vector-Jacobian products.  Each lane accumulates its
-// parameter gradients in its own LDS column; the workgroup sums the columns
+// parameter gradients in its own LDS column (radius, centre, n_eff) and in two
+// registers (the coatings of the ghost's two reflecting elements); the
+// workgroup sums the columns
 // with a fixed butterfly into partials laid out [parameter][workgroup], which a
 // second kernel reduces per parameter in a fixed order: results are
 // deterministic and independent of scheduling.
@@ -231,64 +233,109 @@ __device__ __forceinline__ void reflect_bwd(f3 i, f3 n, f3 rb, f3& ib, f3& nb) {
     nb = add3(nb, add3(scl3(i, -2 * nr), scl3(rb, -2 * ni)));
 }
 
-// fresnel (raytracing.cl:1-16): returns d/d(theta0, n1, n2) * Rb
-__device__ __forceinline__ void fresnel_bwd(float theta0, float n1, float n2, float Rb, float& t0b, float& n1b,
-                            float& n2b) {
-    const float s0 = sinf(theta0), c0 = cosf(theta0);
-    const float a1 = s0 * n1 / n2;
-    const float th1 = asinf(a1);
-    const float c1 = cosf(th1);
-    const float A = n1 * c0, B = n2 * c1, Cc = n1 * c1, D = n2 * c0;
-    const float rs = (A - B) / (A + B), rp = (Cc - D) / (Cc + D);
+// The Fresnel adjoints run on the two reflection events of a ghost only.  Each
+// evaluates the event's forward terms once, with the forward's operations in
+// the forward's order (contraction off), so the reflectance it returns is
+// bit-identical to fresnel_f / fresnel_ar_f's; the gradient then reuses those
+// terms.  Derivatives take hardware reciprocals and square roots (about
+// 1 ulp), and cosines that the terms already fix come from identities
+// (cos asin a = sqrt(1 - a^2), cos(x +- y) from the sines and cosines of x, y).
+// Kept out of line, returning the reflectance and partials by value.
+struct FresnelGrad {
+    float r, t0, d1, n0, n1, n2;
+};
+
+// fresnel (raytracing.cl:1-16): R and Rb * dR/d(theta0, n1, n2), for R > 0
+__device__ __noinline__ FresnelGrad fresnel_grad(float theta0, float n1, float n2, float Rb) {
+    FresnelGrad g{0, 0, 0, 0, 0, 0};
+    float s0, c0, a1, th1, c1, A, B, Cc, D, rs, rp;
+    {
+#pragma clang fp contract(off)
+        s0 = sinf(theta0);
+        a1 = s0 * n1 / n2;
+        th1 = asinf(a1);
+        c0 = cosf(theta0);
+        c1 = cosf(th1);
+        A = n1 * c0;
+        B = n2 * c1;
+        Cc = n1 * c1;
+        D = n2 * c0;
+        rs = (A - B) / (A + B);
+        rp = (Cc - D) / (Cc + D);
+        g.r = (rs * rs + rp * rp) / 2;
+    }
+    if (!(g.r > 0)) return g;
     const float rsb = Rb * rs, rpb = Rb * rp;
     // rs = (A - B)/(A + B): d/dA = 2B/(A+B)^2, d/dB = -2A/(A+B)^2
-    const float Ab = rsb * 2 * B / ((A + B) * (A + B));
-    const float Bb = -rsb * 2 * A / ((A + B) * (A + B));
-    const float Cb2 = rpb * 2 * D / ((Cc + D) * (Cc + D));
-    const float Db = -rpb * 2 * Cc / ((Cc + D) * (Cc + D));
-    n1b += Ab * c0 + Cb2 * c1;
-    n2b += Bb * c1 + Db * c0;
-    float c0b = Ab * n1 + Db * n2;
+    const float is = frcp(A + B), ip = frcp(Cc + D);
+    const float Ab = rsb * 2 * B * is * is, Bb = -rsb * 2 * A * is * is;
+    const float Cb2 = rpb * 2 * D * ip * ip, Db = -rpb * 2 * Cc * ip * ip;
+    g.n1 = Ab * c0 + Cb2 * c1;
+    g.n2 = Bb * c1 + Db * c0;
+    const float c0b = Ab * n1 + Db * n2;
     const float c1b = Bb * n2 + Cb2 * n1;
-    const float th1b = -c1b * sinf(th1);
+    const float th1b = -c1b * a1;  // sin(asin a1) = a1
     float s0b = 0.0f;
     const float q = 1 - a1 * a1;
     if (q > 1e-12f) {
-        const float a1b = th1b / sqrtf(q);
-        s0b += a1b * n1 / n2;
-        n1b += a1b * s0 / n2;
-        n2b += -a1b * a1 / n2;
+        const float a1b = th1b * frsq(q), in2 = frcp(n2);
+        s0b += a1b * n1 * in2;
+        g.n1 += a1b * s0 * in2;
+        g.n2 += -a1b * a1 * in2;
     }
-    t0b += s0b * c0 - c0b * s0;
+    g.t0 = s0b * c0 - c0b * s0;
+    return g;
 }
 
-// fresnel_ar (raytracing.cl:18-59): gradient of R w.r.t. theta0, d1, n0, n1, n2
-__device__ __forceinline__ void fresnel_ar_bwd(float theta0, float lambda, float d1, float n0, float n1, float n2,
-                               float Rb, float& t0b, float& d1b, float& n0b, float& n1b,
-                               float& n2b) {
+// fresnel_ar (raytracing.cl:18-59): R and Rb * dR/d(theta0, d1, n0, n1, n2), for R > 0
+__device__ __noinline__ FresnelGrad fresnel_ar_grad(float theta0, float lambda, float d1, float n0,
+                                                    float n1, float n2, float Rb) {
+    FresnelGrad g{0, 0, 0, 0, 0, 0};
     const float pi = 3.14159265358979323846f;
-    const float s0 = sinf(theta0), c0 = cosf(theta0);
-    const float a1 = s0 * n0 / n1, a2 = s0 * n0 / n2;
-    const float th1 = asinf(a1), th2 = asinf(a2);
-    const float dm = theta0 - th1, dp = theta0 + th1;
-    const float sdm = sinf(dm), sdp = sinf(dp), tdm = tanf(dm), tdp = tanf(dp);
-    const float rs01 = -sdm / sdp, rp01 = tdm / tdp;
-    const float s1 = sinf(th1);
-    const float ts01 = 2 * s1 * c0 / sdp;
-    const float cdm = cosf(dm);
-    const float tp01 = ts01 * cdm;
-    const float em = th1 - th2, ep = th1 + th2;
-    const float sem = sinf(em), sep = sinf(ep), tem = tanf(em), tep = tanf(ep);
-    const float rs12 = -sem / sep, rp12 = tem / tep;
-    const float ris = ts01 * ts01 * rs12, rip = tp01 * tp01 * rp12;
-    const float dy = d1 * n1;
-    const float t1 = tanf(th1);
-    const float dx = t1 * dy;
-    const float delay = sqrtf(dx * dx + dy * dy);
-    const float k4 = 4 * pi / lambda;
-    const float ph = k4 * (delay - dx * s0);
-    const float cph = cosf(ph);
-
+    float s0, c0, a1, a2, th1, th2, dm, dp, sdm, sdp, tdm, tdp, cdm, s1, ts01, tp01, em, ep, sem,
+        sep, tem, tep, rs01, rp01, rs12, rp12, ris, rip, dy, t1, dx, delay, k4, ph, cph;
+    {
+#pragma clang fp contract(off)
+        s0 = sinf(theta0);
+        a1 = s0 * n0 / n1;
+        a2 = s0 * n0 / n2;
+        th1 = asinf(a1);
+        th2 = asinf(a2);
+        dm = theta0 - th1;
+        dp = theta0 + th1;
+        sdp = sinf(dp);
+        sdm = sinf(dm);
+        rs01 = -sdm / sdp;
+        tdm = tanf(dm);
+        tdp = tanf(dp);
+        rp01 = tdm / tdp;
+        s1 = sinf(th1);
+        c0 = cosf(theta0);
+        ts01 = 2 * s1 * c0 / sdp;
+        cdm = cosf(dm);
+        tp01 = ts01 * cdm;
+        em = th1 - th2;
+        ep = th1 + th2;
+        sem = sinf(em);
+        sep = sinf(ep);
+        rs12 = -sem / sep;
+        tem = tanf(em);
+        tep = tanf(ep);
+        rp12 = tem / tep;
+        ris = ts01 * ts01 * rs12;
+        rip = tp01 * tp01 * rp12;
+        dy = d1 * n1;
+        t1 = tanf(th1);
+        dx = t1 * dy;
+        delay = sqrtf(dx * dx + dy * dy);
+        k4 = 4 * pi / lambda;
+        ph = k4 * (delay - dx * s0);
+        cph = cosf(ph);
+        const float out_s2 = rs01 * rs01 + ris * ris + 2 * rs01 * ris * cph;
+        const float out_p2 = rp01 * rp01 + rip * rip + 2 * rp01 * rip * cph;
+        g.r = (out_s2 + out_p2) / 2;
+    }
+    if (!(g.r > 0)) return g;
     const float osb = Rb * 0.5f, opb = Rb * 0.5f;
     const float rs01b = osb * (2 * rs01 + 2 * ris * cph);
     const float risb = osb * (2 * ris + 2 * rs01 * cph);
@@ -301,70 +348,60 @@ __device__ __forceinline__ void fresnel_ar_bwd(float theta0, float lambda, float
     float s0b = -phb * k4 * dx;
     float dyb = 0.0f;
     if (delay > 0) {
-        dxb += delayb * dx / delay;
-        dyb += delayb * dy / delay;
+        const float idl = frcp(delay);
+        dxb += delayb * dx * idl;
+        dyb += delayb * dy * idl;
     }
     const float t1b = dxb * dy;
     dyb += dxb * t1;
-    d1b += dyb * n1;
-    n1b += dyb * d1;
+    g.d1 += dyb * n1;
+    g.n1 += dyb * d1;
     float th1b = t1b * (1 + t1 * t1);
     const float tp01b = ripb * 2 * tp01 * rp12;
     const float rp12b = ripb * tp01 * tp01;
     float ts01b = risb * 2 * ts01 * rs12;
     const float rs12b = risb * ts01 * ts01;
-    const float temb = rp12b / tep, tepb = -rp12b * tem / (tep * tep);
-    const float semb = -rs12b / sep, sepb = rs12b * sem / (sep * sep);
-    const float emb = temb * (1 + tem * tem) + semb * cosf(em);
-    const float epb = tepb * (1 + tep * tep) + sepb * cosf(ep);
+    const float itep = frcp(tep), isep = frcp(sep);
+    const float temb = rp12b * itep, tepb = -rp12b * tem * itep * itep;
+    const float semb = -rs12b * isep, sepb = rs12b * sem * isep * isep;
+    // th1, th2 lie in [-pi/2, pi/2]: cos th = sqrt(1 - a^2), sin th = a
+    const float q1 = 1 - a1 * a1, q2 = 1 - a2 * a2;
+    const float c1 = fsqrt(fmaxf(q1, 0.0f)), c2 = fsqrt(fmaxf(q2, 0.0f));
+    const float cem = c1 * c2 + a1 * a2, cep = c1 * c2 - a1 * a2;
+    const float emb = temb * (1 + tem * tem) + semb * cem;
+    const float epb = tepb * (1 + tep * tep) + sepb * cep;
     th1b += emb + epb;
-    float th2b = -emb + epb;
+    const float th2b = -emb + epb;
     ts01b += tp01b * cdm;
     const float cdmb = tp01b * ts01;
-    float dmb = -cdmb * sinf(dm);
-    const float s1b = ts01b * 2 * c0 / sdp;
-    float c0b = ts01b * 2 * s1 / sdp;
-    float sdpb = -ts01b * ts01 / sdp;
-    th1b += s1b * cosf(th1);
-    const float tdmb = rp01b / tdp, tdpb = -rp01b * tdm / (tdp * tdp);
-    const float sdmb = -rs01b / sdp;
-    sdpb += rs01b * sdm / (sdp * sdp);
-    dmb += tdmb * (1 + tdm * tdm) + sdmb * cosf(dm);
-    const float dpb = tdpb * (1 + tdp * tdp) + sdpb * cosf(dp);
+    float dmb = -cdmb * sdm;
+    const float isdp = frcp(sdp), itdp = frcp(tdp);
+    const float s1b = ts01b * 2 * c0 * isdp;
+    const float c0b = ts01b * 2 * s1 * isdp;
+    float sdpb = -ts01b * ts01 * isdp;
+    th1b += s1b * c1;
+    const float tdmb = rp01b * itdp, tdpb = -rp01b * tdm * itdp * itdp;
+    const float sdmb = -rs01b * isdp;
+    sdpb += rs01b * sdm * isdp * isdp;
+    const float cdp = c0 * c1 - s0 * a1;  // cos(theta0 + th1)
+    dmb += tdmb * (1 + tdm * tdm) + sdmb * cdm;
+    const float dpb = tdpb * (1 + tdp * tdp) + sdpb * cdp;
     float t0 = dmb + dpb;
     th1b += -dmb + dpb;
-    const float q2 = 1 - a2 * a2;
     if (q2 > 1e-12f) {
-        const float a2b = th2b / sqrtf(q2);
-        s0b += a2b * n0 / n2;
-        n0b += a2b * s0 / n2;
-        n2b += -a2b * a2 / n2;
+        const float a2b = th2b * frsq(q2), in2 = frcp(n2);
+        s0b += a2b * n0 * in2;
+        g.n0 += a2b * s0 * in2;
+        g.n2 += -a2b * a2 * in2;
     }
-    const float q1 = 1 - a1 * a1;
     if (q1 > 1e-12f) {
-        const float a1b = th1b / sqrtf(q1);
-        s0b += a1b * n0 / n1;
-        n0b += a1b * s0 / n1;
-        n1b += -a1b * a1 / n1;
+        const float a1b = th1b * frsq(q1), in1 = frcp(n1);
+        s0b += a1b * n0 * in1;
+        g.n0 += a1b * s0 * in1;
+        g.n1 += -a1b * a1 * in1;
     }
     t0 += s0b * c0 - c0b * s0;
-    t0b += t0;
-}
-
-// The Fresnel adjoints run on the two reflection events of a ghost only: kept
-// out of line, returning their partials by value.
-struct FresnelGrad {
-    float t0, d1, n0, n1, n2;
-};
-__device__ __noinline__ FresnelGrad fresnel_grad(float theta0, float n1, float n2, float Rb) {
-    FresnelGrad g{0, 0, 0, 0, 0};
-    fresnel_bwd(theta0, n1, n2, Rb, g.t0, g.n1, g.n2);
-    return g;
-}
-__device__ __noinline__ FresnelGrad fresnel_ar_grad(float theta0, float lambda, float d1, float n0,
-                                                    float n1, float n2, float Rb) {
-    FresnelGrad g{0, 0, 0, 0, 0};
-    fresnel_ar_bwd(theta0, lambda, d1, n0, n1, n2, Rb, g.t0, g.d1, g.n0, g.n1, g.n2);
+    g.t0 = t0;
     return g;
 }
 
@@ -433,10 +470,34 @@ __device__ __forceinline__ float wave_sum(float v) {
     return v;
 }
 
-// per-lane accumulation of one parameter's gradient: column `lane` of the
-// workgroup's [parameter][lane] array, summed over lanes once at the end
-__device__ __forceinline__ void acc(float* s_lacc, int idx, float v) {
-    s_lacc[idx * ADJ_THREADS + threadIdx.x] += v;
+// Per-lane parameter adjoints, summed over the wave once at the end.  Radius,
+// centre and n_eff of every element: column `lane` of the workgroup's LDS
+// array [3 L][lane] (rows radius 0..L-1, centre L..2L-1, n_eff 2L..3L-1).  A
+// ghost reflects at exactly two elements (b1, then b2), so its coating
+// adjoints are two registers.
+struct Grad {
+    float* lacc;
+    float coat_b1, coat_b2;
+};
+static constexpr int ADJ_LDS_ROWS = 3;  // LDS rows per element
+__device__ __forceinline__ void acc(Grad& G, int idx, float v) {
+    G.lacc[idx * ADJ_THREADS + threadIdx.x] += v;
+}
+__device__ __forceinline__ void grad_clear(Grad& G, int L) {
+    for (int i = 0; i < ADJ_LDS_ROWS * L; i++) G.lacc[i * ADJ_THREADS + threadIdx.x] = 0.0f;
+    G.coat_b1 = 0.0f;
+    G.coat_b2 = 0.0f;
+}
+// Wave sum of parameter i in the order [radius L][centre L][coating L][n_eff L].
+__device__ __forceinline__ float grad_sum(const Grad& G, const Path& P, int i) {
+    const int L = P.L;
+    if (i < 2 * L) return wave_sum(G.lacc[i * ADJ_THREADS + threadIdx.x]);
+    if (i >= 3 * L) return wave_sum(G.lacc[(i - L) * ADJ_THREADS + threadIdx.x]);
+    const int e = i - 2 * L;
+    float v = 0.0f;
+    if (e == P.b1) v += wave_sum(G.coat_b1);
+    if (e == P.b2) v += wave_sum(G.coat_b2);
+    return v;
 }
 
 __device__ __forceinline__ void st_store(float* scratch, long N, long ray, int slot, const St& s) {
@@ -462,7 +523,7 @@ __device__ __forceinline__ St st_load(const float* scratch, long N, long ray, in
 // before the event and p its hit point (the position after it); (posb, dirb,
 // reflb) the adjoint of the state after the event on entry and before it on
 // return.  Parameter adjoints go to the lane's LDS column.
-__device__ __forceinline__ void event_rev(const LensF& Lf, const float* s_n, float* s_lacc,
+__device__ __forceinline__ void event_rev(const LensF& Lf, const float* s_n, Grad& G,
                                           const Path& P, int k, const St& in, f3 p, f3& posb,
                                           f3& dirb, float& reflb) {
 #pragma clang fp contract(off)
@@ -493,16 +554,17 @@ __device__ __forceinline__ void event_rev(const LensF& Lf, const float* s_n, flo
                 const float q = sqrtf(n1 * n2);
                 const float nc = fmaxf(q, Lf.cmin);
                 const float d1 = cf / (4.0f * nc);
-                r = fresnel_ar_f(theta, Lf.wavelength, d1, n1, nc, n2);
+                const FresnelGrad fg =
+                    fresnel_ar_grad(theta, Lf.wavelength, d1, n1, nc, n2, reflb * in.refl);
+                r = fg.r;
                 if (r > 0) {
-                    const FresnelGrad fg =
-                        fresnel_ar_grad(theta, Lf.wavelength, d1, n1, nc, n2, reflb * in.refl);
                     t0b = fg.t0;
                     n1b = fg.n0;
                     n2b = fg.n2;
                     const float d1b = fg.d1;
                     float ncb = fg.n1;
-                    acc(s_lacc, 2 * L + el, d1b / (4.0f * nc));
+                    if (k == P.b1) G.coat_b1 += d1b / (4.0f * nc);
+                    else G.coat_b2 += d1b / (4.0f * nc);
                     ncb += -d1b * d1 / nc;
                     if (q > Lf.cmin && q > 0) {
                         n1b += ncb * n2 / (2 * q);
@@ -510,9 +572,9 @@ __device__ __forceinline__ void event_rev(const LensF& Lf, const float* s_n, flo
                     }
                 }
             } else {
-                r = fresnel_f(theta, n1, n2);
+                const FresnelGrad fg = fresnel_grad(theta, n1, n2, reflb * in.refl);
+                r = fg.r;
                 if (r > 0) {
-                    const FresnelGrad fg = fresnel_grad(theta, n1, n2, reflb * in.refl);
                     t0b = fg.t0;
                     n1b = fg.n1;
                     n2b = fg.n2;
@@ -527,15 +589,15 @@ __device__ __forceinline__ void event_rev(const LensF& Lf, const float* s_n, flo
             n1b += ob * in2;
             n2b += -ob * n1 * in2 * in2;
         }
-        acc(s_lacc, 3 * L + el, n2b);
-        if (inrange) acc(s_lacc, 3 * L + n_index, n1b);
+        acc(G, 2 * L + el, n2b);
+        if (inrange) acc(G, 2 * L + n_index, n1b);
     }
     f3 posb_in = mk3(0, 0, 0);
     hit_bwd(h, in.pos, in.dir, p, Ei.radius, Ei.center, posb, nb, ib, posb_in, dirb_in, Rb, Cb);
     posb = posb_in;
     dirb = dirb_in;
-    acc(s_lacc, el, Rb);
-    acc(s_lacc, L + el, Cb);
+    acc(G, el, Rb);
+    acc(G, L + el, Cb);
 }
 
 // init_ray (raytracing.cl:178-215) of grid ray `ray_id`: the vertical probe
@@ -598,7 +660,7 @@ __device__ __forceinline__ bool forward_ray(const LensF& Lf, const float* s_n, c
 // backwards and rebuilt from their checkpoints (the last one too unless its
 // states are still in st[]); the init_ray step closes the chain.
 template <int K>
-__device__ __forceinline__ void reverse_ray(const LensF& Lf, const float* s_n, float* s_lacc,
+__device__ __forceinline__ void reverse_ray(const LensF& Lf, const float* s_n, Grad& G,
                                             const Path& P, const RayInit& ri, const St& fin,
                                             St (&st)[K], const float* scratch, long N, long sidx,
                                             bool last_in_registers, f3 posb, f3 dirb, float reflb) {
@@ -621,7 +683,7 @@ __device__ __forceinline__ void reverse_ray(const LensF& Lf, const float* s_n, f
 #pragma unroll
         for (int j = K - 1; j >= 0; j--) {
             if (k0 + j < P.E) {
-                event_rev(Lf, s_n, s_lacc, P, k0 + j, st[j], hit, posb, dirb, reflb);
+                event_rev(Lf, s_n, G, P, k0 + j, st[j], hit, posb, dirb, reflb);
                 hit = st[j].pos;
             }
         }
@@ -634,8 +696,8 @@ __device__ __forceinline__ void reverse_ray(const LensF& Lf, const float* s_n, f
         const Hit h0 = hit_at(ri.down, ri.I0.p, e0.radius, e0.center);
         hit_bwd(h0, ri.gpt, ri.down, ri.I0.p, e0.radius, e0.center, posb, mk3(0, 0, 0), 0.0f, gpb,
                 ddb, Rb, Cb);
-        acc(s_lacc, 0, Rb);
-        acc(s_lacc, P.L, Cb);
+        acc(G, 0, Rb);
+        acc(G, P.L, Cb);
     }
 }
 
@@ -662,7 +724,7 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_trace_adjoint(
     float* __restrict__ scratch, float* __restrict__ partial, float* __restrict__ fwd_out) {
 #pragma clang fp contract(off)
     __shared__ float s_n[RF_LMAX];
-    extern __shared__ float s_lacc[];  // [4 L][ADJ_THREADS]
+    extern __shared__ float s_lacc[];  // [ADJ_LDS_ROWS L][ADJ_THREADS]
     const int L = prm.lens_count;
     const int w = blockIdx.y, p = blockIdx.z;
     const int g = prm.grid_count, R = g * g, W = prm.wavelength_count;
@@ -674,8 +736,8 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_trace_adjoint(
         if (disperse && n > 1) n = dispersion_f(wavelength, e.coefficients);
         s_n[threadIdx.x] = n;
     }
-    if (MODE != ADJ_FORWARD)
-        for (int i = 0; i < 4 * L; i++) s_lacc[i * ADJ_THREADS + threadIdx.x] = 0.0f;
+    Grad G{s_lacc, 0.0f, 0.0f};
+    if (MODE != ADJ_FORWARD) grad_clear(G, L);
     __syncthreads();
 
     const long N = (long)prm.path_count * W * R;
@@ -716,12 +778,12 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_trace_adjoint(
     }
     // reverse (rays that do not reach the sensor contribute nothing)
     if (alive)
-        reverse_ray<K>(Lf, s_n, s_lacc, P, ri, s, st, scratch, N, idx, MODE == ADJ_FUSED,
+        reverse_ray<K>(Lf, s_n, G, P, ri, s, st, scratch, N, idx, MODE == ADJ_FUSED,
                        mk3(gx[idx], gy[idx], 0.0f), mk3(0, 0, 0), gr[idx]);
     const long blk = ((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
     const long nblk = (long)gridDim.x * gridDim.y * gridDim.z;
     for (int i = 0; i < 4 * L; i++) {
-        const float v = wave_sum(s_lacc[i * ADJ_THREADS + threadIdx.x]);
+        const float v = grad_sum(G, P, i);
         if (threadIdx.x == 0) partial[i * nblk + blk] = v;
     }
 }
@@ -841,7 +903,7 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_trace_fit(FitArgs a) {
 #pragma clang fp contract(off)
     __shared__ float s_n[RF_LMAX];
     __shared__ uint64_t s_chunk;
-    extern __shared__ float s_lacc[];  // [4 L][ADJ_THREADS]
+    extern __shared__ float s_lacc[];  // [ADJ_LDS_ROWS L][ADJ_THREADS]
     const int lane = threadIdx.x;
     const uint64_t total = a.chunk_offsets[a.nseg];
     float* scratch = a.scratch + (long)blockIdx.x * a.slots * ADJ_FIELDS * ADJ_THREADS;
@@ -873,7 +935,8 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_trace_fit(FitArgs a) {
             if (disperse && n > 1) n = dispersion_f(wavelength, e.coefficients);
             s_n[lane] = n;
         }
-        for (int i = 0; i < 4 * L; i++) s_lacc[i * ADJ_THREADS + lane] = 0.0f;
+        Grad G{s_lacc, 0.0f, 0.0f};
+        grad_clear(G, L);
         __syncthreads();
         LensF Lf{lens, a.coatings + first, a.cmin, wavelength};
         Path P;
@@ -909,14 +972,14 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_trace_fit(FitArgs a) {
             const float lx = fit_term(dx, a.huber, &gx), ly = fit_term(dy, a.huber, &gy);
             const float lr = fit_term(dr, a.huber, &gr);
             loss += a.loss_scale * (lx + ly + lr);
-            reverse_ray<K>(Lf, s_n, s_lacc, P, ri, s, st, scratch, ADJ_THREADS, lane, true,
+            reverse_ray<K>(Lf, s_n, G, P, ri, s, st, scratch, ADJ_THREADS, lane, true,
                            mk3(a.loss_scale * gx, a.loss_scale * gy, 0.0f), mk3(0, 0, 0),
                            a.loss_scale * gr * a.inv_refl_scale);
         }
         float* out = a.partial + (long)c * pstride;
         for (int b = 0; b < 4; b++) {
             for (int e = 0; e < a.LF; e++) {
-                const float v = e < L ? wave_sum(s_lacc[(b * L + e) * ADJ_THREADS + lane]) : 0.0f;
+                const float v = e < L ? grad_sum(G, P, b * L + e) : 0.0f;
                 if (lane == 0) out[b * a.LF + e] = v;
             }
         }
@@ -1062,7 +1125,7 @@ static int adj_launch(int mode, const rf_lens_element* lens, const float* coatin
     char* w = (char*)workspace;
     dim3 grid(A.bx, prm->wavelength_count, prm->path_count);
     const size_t lds =
-        mode == ADJ_FORWARD ? 0 : (size_t)4 * prm->lens_count * ADJ_THREADS * sizeof(float);
+        mode == ADJ_FORWARD ? 0 : (size_t)ADJ_LDS_ROWS * prm->lens_count * ADJ_THREADS * sizeof(float);
     hipLaunchKernelGGL(adj_kernel(K, mode), grid, dim3(ADJ_THREADS), lds, s, lens, coating_lambda,
                        paths, wavelengths, *prm, gx, gy, gr, (float*)(w + A.scratch),
                        (float*)(w + A.partial), forward_out);
@@ -1196,7 +1259,7 @@ extern "C" int rf_trace_fit(const rf_lens_element* lenses, const float* coatings
     a.loss_scale = loss_scale;
     a.huber = huber_delta;
     hipLaunchKernelGGL(k_queue_reset_fit, dim3(1), dim3(1), 0, s, a.head);
-    const size_t lds = (size_t)4 * max_elements * ADJ_THREADS * sizeof(float);
+    const size_t lds = (size_t)ADJ_LDS_ROWS * max_elements * ADJ_THREADS * sizeof(float);
     hipLaunchKernelGGL(kern, dim3(workgroups), dim3(ADJ_THREADS), lds, s, a);
     const int nout = (3 + wavelength_count) * max_elements + 1;
     hipLaunchKernelGGL(k_fit_reduce, dim3(nout, lens_instances), dim3(RED_THREADS), 0, s, a.partial,
