@@ -17,7 +17,7 @@ from .batch import BatchRenderer, Tuning
 
 RAYS_PER_LANE = (1, 2, 4)
 WORKGROUPS_PER_CU = (4, 8, 16)
-SUPER_BLOCKS = (2, 4, 8)
+SUPER_BLOCKS = (1, 2, 4, 8)  # bin edges of 8, 16, 32 and 64 px
 FRAMES_PER_PASS = (1, 2, 4, 8)
 
 

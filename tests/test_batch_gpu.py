@@ -108,7 +108,7 @@ def test_batch_frame_matches_oracle(oracle_lib):
 def test_batch_passes_and_super_block_do_not_change_images():
     wl, pos, grids = _setup(frames=4, seed=9)
     outs = []
-    for fpp, sb in ((4, 4), (1, 2), (3, 8)):
+    for fpp, sb in ((4, 4), (1, 2), (3, 8), (2, 1)):
         br = BatchRenderer(wl.project, wl.lens_model, wl.paths, wl.wavelengths, pos, grids,
                            Tuning(frames_per_pass=fpp, super_block=sb))
         br.fit_capacity()
@@ -139,3 +139,22 @@ def test_footprint_grids_range():
     assert g.shape == (4, len(wl.paths))
     assert g.min() >= 8 and g.max() <= 64
     assert len(np.unique(g)) > 1  # ghosts differ in size
+
+
+def test_autotune_choice_is_recorded_and_keeps_the_images(tmp_path):
+    from realflare_amd import autotune
+
+    wl, pos, grids = _setup(frames=2, seed=4)
+    br = BatchRenderer(wl.project, wl.lens_model, wl.paths, wl.wavelengths, pos, grids)
+    br.fit_capacity()
+    ref = br.render().cpu().numpy()
+    path = str(tmp_path / 'autotune.json')
+    best = autotune.tune(br, reps=1, path=path, key='small')
+    assert best.rays_per_lane in autotune.RAYS_PER_LANE
+    assert best.super_block in autotune.SUPER_BLOCKS
+    assert best.frames_per_pass in autotune.FRAMES_PER_PASS
+    rec = br.autotune_record
+    assert len(rec['raster_ms_per_frame']) == len(autotune.SUPER_BLOCKS) * 2  # F = 2: fpp 1, 2
+    assert all(v > 0 for v in rec['trace_ms'].values())
+    assert autotune.load('small', path) == best
+    np.testing.assert_array_equal(br.render().cpu().numpy(), ref)
