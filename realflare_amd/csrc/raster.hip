@@ -904,6 +904,38 @@ __device__ __forceinline__ bool cover_small(int px, int py, const Edges& er) {
     return cover_terms(w01, w12, w23, w30, w20) & 1u;
 }
 
+// cover_small_mask at the kAA samples of every pixel: sample s of the pixel
+// whose sample-grid origin is (px, py) lies at (px + s, py + sub_offset(kAA -
+// 1 + s)) (rasterizing.cl:627-634).  Edge k there is the row at the origin
+// plus the wave-uniform a_k oy + nb_k s, modulo 2^32 like the row itself, so
+// the rows are evaluated once per pixel and a sample costs one add and one
+// comparison per edge.  Returns the lanes with a covered sample and sets
+// *hits to this lane's number of covered samples.  Every lane must be active.
+template <int kAA>
+__device__ __forceinline__ uint64_t cover_small_samples(int px, int py, const Edges& er, int lane,
+                                                        int* hits) {
+    uint32_t base[5];
+#pragma unroll
+    for (int k = 0; k < 5; k++) base[k] = (uint32_t)edge_row(er, k, px, py);
+    uint64_t any = 0;
+    int n = 0;
+#pragma unroll
+    for (int s = 0; s < kAA; s++) {
+        const uint32_t oy = (uint32_t)sub_offset(kAA - 1 + s);
+        uint64_t w[5];
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            const uint32_t d = (uint32_t)er.a[k] * oy + (uint32_t)er.nb[k] * (uint32_t)s;
+            w[k] = __ballot((int)(base[k] + d) >= 0);
+        }
+        const uint64_t ms = cover_terms(w[0], w[1], w[2], w[3], w[4]);
+        any |= ms;
+        n += (int)((ms >> lane) & 1u);
+    }
+    *hits = n;
+    return any;
+}
+
 // rasterizing.cl:95-148 (mean value coordinates of a quad), shading precision:
 // hardware reciprocal square root, reciprocal and fused multiply-adds (the
 // weights only scale a fragment's colour; coverage is decided elsewhere in
@@ -1108,10 +1140,12 @@ static constexpr uint32_t TOUCH = 2;
 // to their pixel lanes slot by slot, i.e. in pair order, so each pixel adds
 // exactly the terms of the per-pixel loop in the same order.  Shading thus runs
 // on full waves even when a quad covers few pixels of the tile.
-// kMask: one sample per pixel and bins made of whole tiles (G.AA == 1,
-// G.bin_exact), the common configuration; its instantiation carries neither
-// the bin re-test nor the multi-sample loops, and so holds fewer registers.
-template <bool kMask>
+// kMask: bins made of whole tiles (G.bin_exact, every bin size that is a
+// multiple of 8), with kAA = G.AA samples per pixel known at compile time: its
+// instantiations carry no bin re-test, and the coverage of a FLAG_SMALL quad
+// step is computed as lane masks, one per sample.  kMask false is the generic
+// form (any bin size, G.AA read at run time, kAA unused).
+template <bool kMask, int kAA>
 __global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ bounds,
                                                const float* __restrict__ ghost_pad,
                                                const Setup* __restrict__ setup,
@@ -1142,7 +1176,7 @@ __global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ bounds
     const uint32_t tile = (uint32_t)frame * (uint32_t)(G.ntx * G.nty) + (uint32_t)(ty * G.ntx + tx);
     const int x = tx * TILE + (lane & (TILE - 1)), y = ty * TILE + (lane / TILE);
     const bool inside = x < G.width && y < G.height;
-    const int AA = kMask ? 1 : G.AA;
+    const int AA = kMask ? kAA : G.AA;
     const int px = x * AA, py = y * AA;
     float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f;
     // queue state (wave-uniform); slot s (the s-th queued pair) in s_slot[s]
@@ -1210,16 +1244,21 @@ __global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ bounds
         const Edges er = edges[e];
         int hits = 0;
         uint64_t m;
-        // one sample per pixel and bins made of whole tiles: the coverage of a
-        // FLAG_SMALL quad step is computed as a lane mask (cover_small_mask)
+        // bins made of whole tiles: the coverage of a FLAG_SMALL quad step is
+        // computed as lane masks (cover_small_mask; with several samples per
+        // pixel, one mask per sample)
         if (kMask && (er.flags & FLAG_SMALL)) {
             if (j + TOUCH < jn) {
                 const uint32_t en = (uint32_t)__builtin_amdgcn_readlane((int)ids, (int)(j + TOUCH));
                 asm volatile("" ::"s"(touched));
                 touched = reinterpret_cast<const uint32_t*>(edges)[(size_t)en * (sizeof(Edges) / 4)];
             }
-            m = cover_small_mask(px, py, er) & inside_mask;
-            hits = 1;  // read by the covered lanes only
+            if (kAA == 1) {
+                m = cover_small_mask(px, py, er) & inside_mask;
+                hits = 1;  // read by the covered lanes only
+            } else {
+                m = cover_small_samples<kAA>(px, py, er, lane, &hits) & inside_mask;
+            }
         } else {
             bool ok = inside;
             if (!kMask && ok && !G.bin_exact) {
@@ -1494,14 +1533,19 @@ static int draw_impl(const float4* vtx, const float4* bounds, const float* ghost
     const long texels = (long)(G.gw + 2 * GHOST_PAD) * (G.gh + 2 * GHOST_PAD);
     hipLaunchKernelGGL(k_pad_ghost, dim3((unsigned)min(blocks_for(texels, 256), 4096u)), dim3(256), 0,
                        s, ghost, G.gw, G.gh, r.ghost_pad);
-    if (G.AA == 1 && G.bin_exact)
-        hipLaunchKernelGGL(k_raster<true>, dim3((unsigned)raster_grid(G)), dim3(64), 0, s, bounds,
-                           r.ghost_pad, r.setup, r.edges, r.attr, r.vals, r.tstart, r.tend,
-                           r.order_vals, r.h, G, image);
-    else
-        hipLaunchKernelGGL(k_raster<false>, dim3((unsigned)raster_grid(G)), dim3(64), 0, s, bounds,
-                           r.ghost_pad, r.setup, r.edges, r.attr, r.vals, r.tstart, r.tend,
-                           r.order_vals, r.h, G, image);
+    decltype(&k_raster<false, 0>) kern = k_raster<false, 0>;
+    if (G.bin_exact) {
+        switch (G.AA) {
+            case 1: kern = k_raster<true, 1>; break;
+            case 2: kern = k_raster<true, 2>; break;
+            case 4: kern = k_raster<true, 4>; break;
+            case 8: kern = k_raster<true, 8>; break;
+            default: break;
+        }
+    }
+    hipLaunchKernelGGL(kern, dim3((unsigned)raster_grid(G)), dim3(64), 0, s, bounds, r.ghost_pad,
+                       r.setup, r.edges, r.attr, r.vals, r.tstart, r.tend, r.order_vals, r.h, G,
+                       image);
     RF_LAUNCH_CHECK("rasterizer");
     return RF_OK;
 }

@@ -55,29 +55,32 @@ __global__ __launch_bounds__(256) void k_starburst(const float* __restrict__ spe
     const int lambda_delta = 730 - 390;
     float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f, r3 = 0.0f;
     const float fx = (float)px, fy = (float)py;
-    // No blur and no rotation (the defaults): the jitter radius and the
-    // rotation angle are exactly 0, so the position is nx, ny scaled by the
-    // wavelength (cos 0 = 1 and sin 0 = 0 leave it unchanged up to the sign of
-    // a zero, which the texture coordinate 0.5 * x + 0.5 does not see), and the
-    // six noise and trigonometric evaluations per sample are skipped.
-    const bool still = p.blur == 0.0f && p.rotation == 0.0f && p.rotation_weight >= 0.0f;
+    // With rotation 0 (the default, and a weight >= 0, so that the power
+    // below is finite) the rotation angle is exactly +-0: cos = 1 and
+    // sin = +-0 leave the position unchanged up to the sign of a zero, which
+    // the texture coordinate 0.5 * x + 0.5 does not see, so the two noise,
+    // power and trigonometric evaluations of the rotation are skipped.  With
+    // blur 0 (the default) the jitter radius is exactly 0 and the jitter's two
+    // noise and trigonometric evaluations are skipped as well.  Both skips
+    // are bit-identical to evaluating them.
+    const bool jitter = p.blur != 0.0f;
+    const bool rotate = !(p.rotation == 0.0f && p.rotation_weight >= 0.0f);
     for (int t = 0; t < samples; ++t) {
         const float step = (float)t / (float)samples;
         const float wavelength = step * (float)lambda_delta + 390.0f;
         const float sc = fmaxf(1 - (1 - (wavelength / 560.0f)) * 1.0f, FLT_MIN);
-        float rx, ry;
-        if (still) {
-            rx = nx / sc;
-            ry = ny / sc;
-        } else {
-            const float seed = (float)(t * 4);
-            float x = nx, y = ny;
+        const float seed = (float)(t * 4);
+        float x = nx, y = ny;
+        if (jitter) {
             const float radius = p.blur * sqrtf(sb_noise(fx, fy, seed));
             const float alpha = sb_noise(fx, fy, seed + 1) * 2 * pi;
             x += cosf(alpha) * radius;
             y += sinf(alpha) * radius;
-            x = x / sc;
-            y = y / sc;
+        }
+        x = x / sc;
+        y = y / sc;
+        float rx = x, ry = y;
+        if (rotate) {
             const float sign = (sb_noise(fx, fy, seed + 2) > 0.5f) ? 1.0f : -1.0f;
             const float angle =
                 sign * p.rotation * (1.0f - powf(sb_noise(fx, fy, seed + 3), p.rotation_weight));

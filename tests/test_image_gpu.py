@@ -191,7 +191,7 @@ def test_power_spectrum_and_ghost_match_oracle(oracle_lib):
         assert err <= 1e-4
 
 
-@pytest.mark.parametrize('jitter', [False, True])
+@pytest.mark.parametrize('jitter', [False, True, 'blur'])
 def test_starburst_matches_oracle(jitter, oracle_lib):
     from realflare_amd import native
     from realflare_amd.api.color import light_spectrum
@@ -203,7 +203,9 @@ def test_starburst_matches_oracle(jitter, oracle_lib):
     spec = oracle_lib.power_spectrum(ap.astype(np.float64), 1.0)
     cfg = Flare.Starburst()
     cfg.vignetting_enabled = True
-    if jitter:
+    if jitter == 'blur':  # the reference benchmark project's setting
+        cfg.blur, cfg.rotation, cfg.rotation_weight = 1.0, 0.0, 1.0
+    elif jitter:
         cfg.blur, cfg.rotation, cfg.rotation_weight = 1.0, 10.0, 2.0
     prm = starburst_params(cfg, (200, 150), 24, (0.3, 0.2), (1.0, 1.0))
     spec_t = torch.from_numpy(spec).cuda()
@@ -218,6 +220,33 @@ def test_starburst_matches_oracle(jitter, oracle_lib):
     print(f'starburst jitter={jitter} rel L1 {err:.2e}', flush=True)
     assert np.abs(ref).sum() > 0
     assert err <= (1e-2 if jitter else 1e-5)
+
+
+@pytest.mark.parametrize('blur', [0.0, 1.0])
+def test_starburst_skipped_rotation_is_exact(blur):
+    """Rotation 0 skips the rotation's noise and trigonometry; a rotation of
+    1e-30 evaluates them and moves no sample coordinate by a representable
+    amount, so the two images are identical."""
+    from realflare_amd import native
+    from realflare_amd.api.color import light_spectrum
+    from realflare_amd.api.data import Flare
+    from realflare_amd.api.tasks.starburst import starburst_params
+
+    n = 128
+    rng = np.random.default_rng(5)
+    spec_t = torch.from_numpy(rng.random((n, n)).astype(np.float32)).cuda()
+    light = torch.from_numpy(light_spectrum()).cuda()
+    outs = []
+    for rotation in (0.0, 1e-30):
+        cfg = Flare.Starburst()
+        cfg.blur, cfg.rotation, cfg.rotation_weight = blur, rotation, 1.0
+        prm = starburst_params(cfg, (96, 80), 40, (0.1, -0.2), (1.0, 1.0))
+        out = torch.empty((80, 96, 4), dtype=torch.float32, device='cuda')
+        native.call('rf_starburst', native.ptr(spec_t), n, n, native.ptr(light), light.shape[0],
+                    ctypes.byref(prm), native.ptr(out), native.stream())
+        outs.append(out.cpu().numpy())
+    assert np.abs(outs[0]).sum() > 0
+    np.testing.assert_array_equal(outs[0], outs[1])
 
 
 def test_engine_flare_starburst_matches_oracle(oracle_lib):

@@ -11,9 +11,11 @@ rf_rasterize, rf_flare) with the oracle's restatement of rasterizing.cl.
 * End to end: oracle trace + oracle raster against GPU trace + GPU raster,
   within 1e-3 relative L1 (the reference's image bar); vertex positions
   that differ in the last bit can move a pixel edge.
-* Variants: 4x anti-aliasing with 3 sub-wavelengths, a bin size that is not
-  a multiple of the 8-pixel tile, one wavelength (the rasterizer's
-  neighbouring-ray quirk), and the pair-capacity overflow/retry path.
+* Variants: 2x / 4x / 8x anti-aliasing with sub-wavelengths, on bins of
+  whole 8x8 tiles (the lane-mask coverage instantiations) and on bins of 36
+  and 20 pixels (the generic one), one wavelength (the rasterizer's
+  neighbouring-ray quirk), and the pair-capacity overflow/retry path; the
+  bin size leaves the image bit-identical at every anti-aliasing level.
 """
 import ctypes
 
@@ -147,7 +149,8 @@ def test_image_error_against_fp64(oracle_lib):
 
 
 @pytest.mark.parametrize('aa,wsub,bin_size,wl', [(4, 3, 64, None), (2, 1, 48, None),
-                                                  (1, 1, 64, [550]), (8, 2, 40, [450, 600])])
+                                                  (1, 1, 64, [550]), (8, 2, 40, [450, 600]),
+                                                  (4, 2, 36, None), (1, 1, 20, None)])
 def test_raster_variants_same_rays(aa, wsub, bin_size, wl, oracle_lib):
     inp = trace_inputs('small', grid_count=24)
     if wl is not None:
@@ -163,6 +166,26 @@ def test_raster_variants_same_rays(aa, wsub, bin_size, wl, oracle_lib):
     print(f'variant rel L1 {err:.3e}', flush=True)
     assert np.abs(oimg).sum() > 0
     assert err <= 1e-6
+
+
+@pytest.mark.parametrize('aa', [1, 2, 4, 8])
+def test_raster_bin_size_does_not_change_the_image(aa):
+    """A pixel adds the fragments of the quads of its bin in binning order and
+    a quad that covers none of its samples adds nothing, so the bin size does
+    not change the image.  Bins of whole 8x8 tiles (64, 8) run the
+    lane-mask coverage instantiation, bins of 36 the generic one: the images
+    are identical."""
+    inp = trace_inputs('small', grid_count=24)
+    print(f'case raster/bin sizes aa={aa}', flush=True)
+    planes = gpu_trace_planes(inp)
+    ghost = np.random.default_rng(2).random((48, 64)).astype(np.float32)
+    imgs = []
+    for bin_size in (64, 36, 8):
+        prm = params_for(inp, (264, 200), ghost.shape, bin_size=bin_size, aa=aa, wsub=2)
+        imgs.append(gpu_flare(planes, ghost, prm)[0])
+    assert np.abs(imgs[0]).sum() > 0
+    for img in imgs[1:]:
+        np.testing.assert_array_equal(img, imgs[0])
 
 
 def test_raster_capacity_overflow_retry(oracle_lib):
