@@ -16,9 +16,43 @@ namespace rf {
 
 static constexpr int SB_MAX_CACHED = 1024;
 
+// sin of a float argument, reduced in double (Cody-Waite with a two-part
+// pi/2): k = the integer nearest x 2/pi; r = (x - k h) - k l with
+// h = double(pi/2), l = pi/2 - h, the first step exact (x and k h are
+// multiples of 2^-52 and the difference is below 1 in magnitude for
+// |x| < 2^31); then sin x = +-sin r or +-cos r by the quadrant k mod 4, with
+// |r| <= pi/4 handled by the single-precision sine and cosine.  The noise's
+// arguments reach 10^6, where the single-precision sine takes its long
+// reduction path; this one costs a handful of double operations instead.
+__device__ __forceinline__ float sin_reduced(float x) {
+    const double kPio2Hi = 1.5707963267948966;     // double nearest pi/2
+    const double kPio2Lo = 6.123233995736766e-17;  // pi/2 - kPio2Hi
+    const double xd = (double)x;
+    const double k = rint(xd * 0.63661977236758134);  // 2/pi
+    const double r = fma(-k, kPio2Lo, fma(-k, kPio2Hi, xd));
+    float sr, cr;
+    sincosf((float)r, &sr, &cr);
+    const int q = (int)(k - 4.0 * floor(k * 0.25));
+    const float v = (q & 1) ? cr : sr;
+    return (q & 2) ? -v : v;
+}
+
+// x / b correctly rounded, from ib = 1 / b correctly rounded (Markstein's
+// correction: q = x ib is within an ulp of x / b, the remainder x - q b is
+// exact with a fused multiply-add, and q + (x - q b) ib rounded once is the
+// correctly rounded quotient).  Valid while no intermediate is subnormal or
+// overflows: here b lies in [0.69, 1.31] (the wavelength scale) and x is 0
+// or at least 2^-100 in magnitude.
+__device__ __forceinline__ float div_rn(float x, float b, float ib) {
+    const float q = x * ib;
+    const float r = __builtin_fmaf(-q, b, x);
+    return __builtin_fmaf(r, ib, q);
+}
+
+// noise (noise.cl:1-5): fract(sin(x 112.9898 + y 179.233 + z 237.212) 43758.5453)
 __device__ __forceinline__ float sb_noise(float x, float y, float z) {
 #pragma clang fp contract(off)
-    float v = sinf(x * 112.9898f + y * 179.233f + z * 237.212f) * 43758.5453f;
+    float v = sin_reduced(x * 112.9898f + y * 179.233f + z * 237.212f) * 43758.5453f;
     return fminf(v - floorf(v), 0x1.fffffep-1f);
 }
 
@@ -27,6 +61,7 @@ __global__ __launch_bounds__(256) void k_starburst(const float* __restrict__ spe
                                                    rf_starburst_params p, float4* __restrict__ out) {
 #pragma clang fp contract(off)
     __shared__ float4 s_xyz[SB_MAX_CACHED];
+    __shared__ float2 s_sc[SB_MAX_CACHED];  // per sample: wavelength scale, its reciprocal
     const float pi = 3.14159265358979323846f;
     const int samples = p.samples;
     const bool cached = samples <= SB_MAX_CACHED;
@@ -39,6 +74,9 @@ __global__ __launch_bounds__(256) void k_starburst(const float* __restrict__ spe
             v.z = sample_bilinear(light + 2, light_n, 1, 4, step, 0.0f, false);
             v.w = sample_bilinear(light + 3, light_n, 1, 4, step, 0.0f, false);
             s_xyz[t] = v;
+            const float wavelength = step * (float)(730 - 390) + 390.0f;
+            const float sc = fmaxf(1 - (1 - (wavelength / 560.0f)) * 1.0f, FLT_MIN);
+            s_sc[t] = make_float2(sc, 1.0f / sc);
         }
         __syncthreads();
     }
@@ -67,8 +105,16 @@ __global__ __launch_bounds__(256) void k_starburst(const float* __restrict__ spe
     const bool rotate = !(p.rotation == 0.0f && p.rotation_weight >= 0.0f);
     for (int t = 0; t < samples; ++t) {
         const float step = (float)t / (float)samples;
-        const float wavelength = step * (float)lambda_delta + 390.0f;
-        const float sc = fmaxf(1 - (1 - (wavelength / 560.0f)) * 1.0f, FLT_MIN);
+        float sc, isc;
+        if (cached) {
+            const float2 c = s_sc[t];
+            sc = c.x;
+            isc = c.y;
+        } else {
+            const float wavelength = step * (float)lambda_delta + 390.0f;
+            sc = fmaxf(1 - (1 - (wavelength / 560.0f)) * 1.0f, FLT_MIN);
+            isc = 1.0f / sc;
+        }
         const float seed = (float)(t * 4);
         float x = nx, y = ny;
         if (jitter) {
@@ -77,8 +123,8 @@ __global__ __launch_bounds__(256) void k_starburst(const float* __restrict__ spe
             x += cosf(alpha) * radius;
             y += sinf(alpha) * radius;
         }
-        x = x / sc;
-        y = y / sc;
+        x = div_rn(x, sc, isc);
+        y = div_rn(y, sc, isc);
         float rx = x, ry = y;
         if (rotate) {
             const float sign = (sb_noise(fx, fy, seed + 2) > 0.5f) ? 1.0f : -1.0f;
