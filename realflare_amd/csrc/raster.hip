@@ -986,7 +986,8 @@ __device__ __forceinline__ void barycentric_fast(int px, int py, const int* x, c
 // 1.2 formula of sample_bilinear), for shading: no branches.  The lookup reads
 // a copy of the texture framed by GHOST_PAD zero texels on every side (row
 // pitch w + 2 GHOST_PAD texels, written by k_pad_ghost).  The texel
-// coordinate is clamped to [-2, size] before the conversion: a clamped
+// coordinate is clamped to [-2, size] (one median-of-three) before the
+// conversion: a clamped
 // coordinate reads only border texels, as the unclamped one would (both
 // texels of an axis lie outside the texture once the coordinate is at or past
 // size, or at or below -2), every texel it reads lies in the frame, so no
@@ -1007,8 +1008,8 @@ __device__ __forceinline__ float ghost_sample(__amdgpu_buffer_rsrc_t tex, float 
                                               float u, float v) {
 #pragma clang fp contract(fast)
     const float fu = floorf(u), fv = floorf(v);
-    const int i0 = (int)fminf(fmaxf(fu, -2.0f), w);
-    const int j0 = (int)fminf(fmaxf(fv, -2.0f), h);
+    const int i0 = (int)__builtin_amdgcn_fmed3f(fu, -2.0f, w);
+    const int j0 = (int)__builtin_amdgcn_fmed3f(fv, -2.0f, h);
     const float a = u - fu, b = v - fv;
     const uint32_t off =
         ((uint32_t)__umul24((uint32_t)(j0 + GHOST_PAD), (uint32_t)pitch) + (uint32_t)(i0 + GHOST_PAD)) * 4u;
