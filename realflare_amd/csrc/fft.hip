@@ -44,19 +44,22 @@ __global__ void k_twiddles(float2* __restrict__ tw, int n) {
     tw[k] = make_float2((float)c, (float)s);
 }
 
-// In-LDS radix-2 transform of `count` sequences of length n stored
-// contiguously at s[seq * n], already in bit-reversed order.
-__device__ __forceinline__ void lds_fft(float2* s, int n, int count, const float2* __restrict__ tw,
-                                        bool inverse) {
+// In-LDS radix-2 transform of `count` sequences of length n = 2^logn stored
+// contiguously at s[seq * n], already in bit-reversed order.  Butterfly b of
+// a stage: sequence b >> (logn - 1), group and offset from the low bits
+// (powers of two throughout, so shifts and masks).
+__device__ __forceinline__ void lds_fft(float2* s, int n, int logn, int count,
+                                        const float2* __restrict__ tw, bool inverse) {
 #pragma clang fp contract(off)
-    const int total = count * (n >> 1);
-    for (int len = 2; len <= n; len <<= 1) {
-        const int half = len >> 1, tstride = n / len;
+    if (logn == 0) return;
+    const int total = count << (logn - 1), hmask = (n >> 1) - 1;
+    for (int lh = 0; lh < logn; lh++) {  // half = 2^lh, len = 2^(lh + 1)
+        const int half = 1 << lh, tshift = logn - 1 - lh;  // tstride = n / len
         for (int b = threadIdx.x; b < total; b += blockDim.x) {
-            const int seq = b / (n >> 1), r = b % (n >> 1);
-            const int grp = r / half, k = r % half;
-            const int i0 = seq * n + grp * len + k, i1 = i0 + half;
-            float2 w = tw[k * tstride];
+            const int seq = b >> (logn - 1), r = b & hmask;
+            const int grp = r >> lh, k = r & (half - 1);
+            const int i0 = (seq << logn) + (grp << (lh + 1)) + k, i1 = i0 + half;
+            float2 w = tw[k << tshift];
             if (inverse) w.y = -w.y;
             const float2 a = s[i0];
             const float2 t = cmul(s[i1], w);
@@ -77,7 +80,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_rows(float2* __restrict__ d
         s[r] = row[i];
     }
     __syncthreads();
-    lds_fft(s, n, 1, tw, inverse != 0);
+    lds_fft(s, n, logn, 1, tw, inverse != 0);
     for (int i = threadIdx.x; i < n; i += blockDim.x)
         row[i] = make_float2(s[i].x * scale, s[i].y * scale);
 }
@@ -96,7 +99,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_cols(float2* __restrict__ d
         s[c * h + r] = data[(long)row * w + c0 + c];
     }
     __syncthreads();
-    lds_fft(s, h, cols, tw, inverse != 0);
+    lds_fft(s, h, logh, cols, tw, inverse != 0);
     for (int i = threadIdx.x; i < total; i += blockDim.x) {
         const int row = i / cols, c = i % cols;
         const float2 v = s[c * h + row];
