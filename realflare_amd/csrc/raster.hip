@@ -971,8 +971,14 @@ __device__ __forceinline__ void barycentric_fast(int px, int py, const int* x, c
         const int j = (i + 1) & 3;
         float a = sx[i] * sy[j] - sx[j] * sy[i];
         const float d = sx[i] * sx[j] + sy[i] * sy[j];
+        // On the line through vertices i and j but outside their segment
+        // (a == 0, d > 0) the term r_i r_j - d is exactly 0, as the source's
+        // correctly rounded lengths of integer vectors give it wherever the
+        // lengths are integers; the rsq-based lengths are not exact, and their
+        // residual over the source's 1e-3 would be a spurious weight.
+        const float num = (a == 0.0f && d > 0.0f) ? 0.0f : r[i] * r[j] - d;
         a = a == 0.0f ? 1e-3f : a;
-        t[i] = (r[i] * r[j] - d) * __builtin_amdgcn_rcpf(a);
+        t[i] = num * __builtin_amdgcn_rcpf(a);
     }
     float ww[4];
 #pragma unroll

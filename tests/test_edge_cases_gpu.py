@@ -1,0 +1,145 @@
+"""Edge cases of the render path against the oracle (raytracing.cl:218-359,
+rasterizing.cl:233-663), at the sizes the reference admits:
+
+* images whose sides are not multiples of the 8-pixel tile (ragged tiles at
+  the right and bottom edges), down to a single pixel, with one and several
+  samples per pixel, on bins of whole tiles and on other bins: within 1e-6
+  relative L1 of the oracle fed the same traced rays (or equal, for an
+  image the oracle leaves black);
+* the smallest grid (2 x 2 rays, one quad) with one wavelength (the
+  rasterizer's read of the next vertex past the buffer), trace and image;
+* a light 89.4 degrees off axis: the rays the trace loses (NaN planes) are
+  the oracle's, and every ghost lands outside the image, which is exactly
+  zero, as the oracle's;
+* planes that are NaN everywhere (no valid quad): an exactly zero image;
+* a narrow image whose sub-pixel quads put pixel centres on edge lines:
+  the image error against the FP64 oracle rasterizer on the same rays stays
+  within twice the FP32 oracle's.
+"""
+import numpy as np
+import pytest
+import torch
+
+from helpers import trace_inputs
+from raster_helpers import (gpu_flare, gpu_trace_planes, make_ghost, oracle_image, params_for,
+                            planes_to_oracle_rays, rel_l1)
+
+pytestmark = pytest.mark.gpu
+
+
+def _compare(img, ref, label):
+    ref = np.asarray(ref, np.float64)
+    if np.abs(ref).sum() == 0:
+        print(f'{label}: oracle image black, GPU max {np.abs(img).max():.3e}', flush=True)
+        assert np.abs(img).max() == 0
+        return
+    err = rel_l1(img, ref)
+    print(f'{label}: rel L1 {err:.3e}', flush=True)
+    assert err <= 1e-6
+
+
+@pytest.mark.parametrize('res,aa,bin_size', [((203, 77), 1, 64), ((203, 77), 2, 36),
+                                             ((9, 300), 4, 64), ((1, 1), 1, 64),
+                                             ((1, 1), 4, 20), ((17, 9), 8, 48)])
+def test_ragged_and_tiny_images_match_oracle(res, aa, bin_size, oracle_lib):
+    inp = trace_inputs('small', grid_count=20)
+    print(f'case edge/ragged res={res} aa={aa} bin={bin_size}', flush=True)
+    planes = gpu_trace_planes(inp)
+    ghost = make_ghost(oracle_lib, res=32)
+    prm = params_for(inp, res, ghost.shape, bin_size=bin_size, aa=aa, wsub=2)
+    rays = planes_to_oracle_rays(planes.cpu().numpy(), oracle_lib)
+    *_, oimg = oracle_image(oracle_lib, rays, inp, prm, ghost)
+    img, _ = gpu_flare(planes, ghost, prm)
+    assert img.shape == (res[1], res[0], 4)
+    assert np.isfinite(img).all()
+    _compare(img, oimg, f'ragged {res} aa={aa} bin={bin_size}')
+
+
+def test_minimum_grid_single_wavelength(oracle_lib):
+    inp = trace_inputs('small', grid_count=2)
+    inp['wavelengths'] = np.array([520], np.int32)
+    inp['grid_length'] = 3.0  # the one quad's rays pass the aperture
+    print('case edge/grid 2 x 2, one wavelength', flush=True)
+    planes = gpu_trace_planes(inp)
+    ref = oracle_lib.trace(inp['lens'], inp['paths'], inp['wavelengths'], inp['aperture_index'],
+                           inp['coating_min_ior'], inp['grid_count'], inp['grid_length'],
+                           inp['direction'], precision='f32')
+    got = planes.cpu().numpy()
+    x_ref = ref['pos'][..., 0].reshape(got.shape[1:])
+    np.testing.assert_array_equal(np.isnan(got[0]), np.isnan(x_ref))
+    ok = ~np.isnan(x_ref)
+    assert ok.any()
+    np.testing.assert_allclose(got[0][ok], x_ref[ok], rtol=1e-5, atol=1e-6)
+    ghost = make_ghost(oracle_lib, res=32)
+    prm = params_for(inp, (64, 48), ghost.shape)
+    rays = planes_to_oracle_rays(got, oracle_lib)
+    *_, oimg = oracle_image(oracle_lib, rays, inp, prm, ghost)
+    img, _ = gpu_flare(planes, ghost, prm)
+    assert np.abs(oimg).sum() > 0  # the quads are in view
+    _compare(img, oimg, 'grid 2 x 2')
+
+
+def test_light_far_off_axis(oracle_lib):
+    inp = trace_inputs('small', grid_count=16)
+    f = float(inp['direction'][2])
+    inp['direction'] = np.array([100.0 * f, 0.0, f, 1.0])  # atan(100): 89.4 degrees off axis
+    print('case edge/light 89.4 degrees off axis', flush=True)
+    planes = gpu_trace_planes(inp)
+    ref = oracle_lib.trace(inp['lens'], inp['paths'], inp['wavelengths'], inp['aperture_index'],
+                           inp['coating_min_ior'], inp['grid_count'], inp['grid_length'],
+                           inp['direction'], precision='f32')
+    got = planes.cpu().numpy()
+    lost = np.isnan(ref['pos'][..., 0].reshape(got.shape[1:]))
+    np.testing.assert_array_equal(np.isnan(got[0]), lost)
+    print(f'light off axis: {lost.mean():.4f} of the rays lost', flush=True)
+    ghost = make_ghost(oracle_lib, res=32)
+    prm = params_for(inp, (128, 96), ghost.shape)
+    rays = planes_to_oracle_rays(got, oracle_lib)
+    *_, oimg = oracle_image(oracle_lib, rays, inp, prm, ghost)
+    img, _ = gpu_flare(planes, ghost, prm)
+    _compare(img, oimg, 'light off axis')
+
+
+def test_all_nan_planes_give_a_zero_image(oracle_lib):
+    inp = trace_inputs('small', grid_count=12)
+    P, W, R = len(inp['paths']), len(inp['wavelengths']), 12 * 12
+    planes = torch.full((6, P, W, R), float('nan'), dtype=torch.float32, device='cuda')
+    ghost = make_ghost(oracle_lib, res=32)
+    prm = params_for(inp, (96, 64), ghost.shape, aa=2)
+    img, _ = gpu_flare(planes, ghost, prm)
+    assert np.array_equal(img, np.zeros_like(img))
+
+
+def test_narrow_image_error_against_fp64(oracle_lib):
+    """A 9-pixel-wide image at 4 samples per pixel: sub-pixel quads whose
+    edge lines pass exactly through pixel centres (a == 0 in the mean value
+    coordinates, rasterizing.cl:95-148).  Same rays into the FP32 oracle, the
+    FP64 oracle rasterizer and the GPU: the GPU's image error against FP64 is
+    within twice the FP32 oracle's (it was 1000x larger while the shading's
+    approximate lengths left a residual on those lines)."""
+    from realflare_amd.api.color import light_spectrum
+
+    inp = trace_inputs('small', grid_count=20)
+    print('case edge/narrow image vs FP64', flush=True)
+    planes = gpu_trace_planes(inp)
+    ghost = make_ghost(oracle_lib, res=32)
+    p = planes.cpu().numpy()
+    rays = planes_to_oracle_rays(p, oracle_lib)
+    r64 = np.zeros(rays.shape, oracle_lib.ray_dtype('f64'))
+    for f in ('pos', 'pos_apt', 'rrel', 'reflectance'):
+        r64[f] = rays[f]
+    g = inp['grid_count']
+    prm = params_for(inp, (9, 300), ghost.shape, bin_size=64, aa=4, wsub=2)
+    *_, o32 = oracle_image(oracle_lib, rays, inp, prm, ghost)
+    b64, i64 = oracle_lib.prim_shader(r64, g, prm.area_orig, prm.min_area, precision='f64')
+    v64 = oracle_lib.vertex_shader(r64, i64, g, prm.screen_transform, (prm.width, prm.height),
+                                   precision='f64')
+    o64 = oracle_lib.rasterize(v64, b64, prm.wavelength_sub_count, g, prm.anti_aliasing,
+                               prm.intensity, prm.ghost_scale, ghost, light_spectrum(),
+                               prm.screen_transform, prm.bin_size, (prm.width, prm.height),
+                               precision='f64')
+    img, _ = gpu_flare(planes, ghost, prm)
+    e_gpu, e_o32 = rel_l1(img, o64), rel_l1(o32, o64)
+    print(f'narrow image vs FP64: gpu {e_gpu:.3e}, fp32 oracle {e_o32:.3e}', flush=True)
+    assert np.abs(o64).sum() > 0
+    assert e_gpu <= 2 * e_o32 + 1e-7

@@ -92,7 +92,7 @@ def test_fft2_matches_numpy():
     # powers of two, and sides of any length (Bluestein), odd, prime, mixed,
     # and the largest of each kind
     for h, w in [(128, 256), (64, 64), (1024, 32), (1000, 600), (37, 12), (5, 1), (97, 128),
-                 (2048, 3), (6, 4096)]:
+                 (2048, 3), (6, 4096), (1, 1), (1, 7), (2, 1), (1, 2)]:
         x = (rng.standard_normal((h, w)) + 1j * rng.standard_normal((h, w))).astype(np.complex64)
         for inverse in (0, 1):
             t = torch.from_numpy(x.view(np.float32).copy()).cuda()
