@@ -12,9 +12,10 @@ rasterizing.cl:233-663), at the sizes the reference admits:
   the oracle's, and every ghost lands outside the image, which is exactly
   zero, as the oracle's;
 * planes that are NaN everywhere (no valid quad): an exactly zero image;
-* a narrow image whose sub-pixel quads put pixel centres on edge lines:
-  the image error against the FP64 oracle rasterizer on the same rays stays
-  within twice the FP32 oracle's.
+* a narrow image whose sub-pixel quads put pixel centres on edge lines, and
+  a light on the optical axis (symmetric ghosts): the image error against
+  the FP64 oracle rasterizer on the same rays stays within twice the FP32
+  oracle's.
 """
 import numpy as np
 import pytest
@@ -142,4 +143,40 @@ def test_narrow_image_error_against_fp64(oracle_lib):
     e_gpu, e_o32 = rel_l1(img, o64), rel_l1(o32, o64)
     print(f'narrow image vs FP64: gpu {e_gpu:.3e}, fp32 oracle {e_o32:.3e}', flush=True)
     assert np.abs(o64).sum() > 0
+    assert e_gpu <= 2 * e_o32 + 1e-7
+
+
+@pytest.mark.parametrize('aa', [1, 4])
+def test_light_on_axis_against_fp64(aa, oracle_lib):
+    """The light on the optical axis: every ghost is centred and symmetric, so
+    pixel centres fall on quad edges and their lines often.  Same rays into
+    the FP32 oracle, the FP64 oracle rasterizer and the GPU: within 1e-6 of
+    the FP32 oracle, and no further from FP64 than twice the FP32 oracle."""
+    from realflare_amd.api.color import light_spectrum
+
+    inp = trace_inputs('small', grid_count=24)
+    f = float(inp['direction'][2])
+    inp['direction'] = np.array([0.0, 0.0, f, 1.0])
+    print(f'case edge/light on axis aa={aa}', flush=True)
+    planes = gpu_trace_planes(inp)
+    ghost = make_ghost(oracle_lib, res=32)
+    rays = planes_to_oracle_rays(planes.cpu().numpy(), oracle_lib)
+    r64 = np.zeros(rays.shape, oracle_lib.ray_dtype('f64'))
+    for fld in ('pos', 'pos_apt', 'rrel', 'reflectance'):
+        r64[fld] = rays[fld]
+    g = inp['grid_count']
+    prm = params_for(inp, (256, 192), ghost.shape, aa=aa, wsub=2)
+    *_, o32 = oracle_image(oracle_lib, rays, inp, prm, ghost)
+    b64, i64 = oracle_lib.prim_shader(r64, g, prm.area_orig, prm.min_area, precision='f64')
+    v64 = oracle_lib.vertex_shader(r64, i64, g, prm.screen_transform, (prm.width, prm.height),
+                                   precision='f64')
+    o64 = oracle_lib.rasterize(v64, b64, prm.wavelength_sub_count, g, prm.anti_aliasing,
+                               prm.intensity, prm.ghost_scale, ghost, light_spectrum(),
+                               prm.screen_transform, prm.bin_size, (prm.width, prm.height),
+                               precision='f64')
+    img, _ = gpu_flare(planes, ghost, prm)
+    e32, e_gpu, e_o32 = rel_l1(img, o32), rel_l1(img, o64), rel_l1(o32, o64)
+    print(f'on axis aa={aa}: gpu-o32 {e32:.3e}, gpu-o64 {e_gpu:.3e}, o32-o64 {e_o32:.3e}', flush=True)
+    assert np.abs(o64).sum() > 0
+    assert e32 <= 1e-6
     assert e_gpu <= 2 * e_o32 + 1e-7
