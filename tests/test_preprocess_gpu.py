@@ -79,3 +79,15 @@ def test_culled_render_equals_render_of_kept_ghosts():
     ref = RasterizingTask().run(p, rays, ghost).array
     np.testing.assert_array_equal(culled, ref)
     assert np.abs(full).sum() > np.abs(culled).sum()
+
+
+def test_full_cull_keeps_no_index_so_every_ghost_renders():
+    """cull_percentage 1.0 keeps no path index, and an empty index tuple means
+    every path to the trace (lens.py:89-113: the filter runs only for a
+    non-empty tuple), so the reference renders every ghost, as at cull 0."""
+    wl, p = _project(1.0)
+    assert PreprocessTask().run(p) == ()
+    everything = _render(p)
+    p.render.cull_percentage = 0.0
+    np.testing.assert_array_equal(everything, _render(p))
+    assert np.abs(everything).sum() > 0
