@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define RF_ABI_VERSION 2  /* 2: rf_trace_fit takes huber_delta */
+#define RF_ABI_VERSION 3  /* 2: rf_trace_fit takes huber_delta; 3: its lost-ray count */
 
 typedef void *rf_stream_t;
 
@@ -217,9 +217,12 @@ int rf_trace_batch(const rf_lens_element *lenses, const int32_t *lens_table,
  *                       for rays that reach the sensor in trace and target, with
  *                       f(r) = r^2, or with huber_delta > 0 and |r| > huber_delta
  *                       f(r) = 2 huber_delta |r| - huber_delta^2
- *   grad_out            float[instances][(3 + W) * max_elements + 1]: d/d radius,
+ *   grad_out            float[instances][(3 + W) * max_elements + 2]: d/d radius,
  *                       d/d centre, d/d coating (max_elements each), d/d ior
- *                       for each wavelength, then the instance's loss
+ *                       for each wavelength, the instance's loss, then the
+ *                       number of its rays that have a target but that the
+ *                       trace loses (stopped, or at a non-finite position;
+ *                       counted exactly, stored as the nearest float)
  *   forward_out         optional float[3][ray_total]: x, y, reflectance (NaN
  *                       where the ray does not reach the sensor)
  * Deterministic: partials are per chunk and reduced in a fixed order.

@@ -469,6 +469,11 @@ __device__ __forceinline__ float wave_sum(float v) {
     for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
     return v;
 }
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += (uint32_t)__shfl_xor((int)v, m, 64);
+    return v;
+}
 
 // Per-lane parameter adjoints, summed over the wave once at the end.  Radius,
 // centre and n_eff of every element: column `lane` of the workgroup's LDS
@@ -907,7 +912,7 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_trace_fit(FitArgs a) {
     const int lane = threadIdx.x;
     const uint64_t total = a.chunk_offsets[a.nseg];
     float* scratch = a.scratch + (long)blockIdx.x * a.slots * ADJ_FIELDS * ADJ_THREADS;
-    const int pstride = 4 * a.LF + 1;
+    const int pstride = 4 * a.LF + 2;
     for (;;) {
         if (lane == 0) s_chunk = atomicAdd((unsigned long long*)a.head, 1ull);
         __syncthreads();
@@ -946,6 +951,7 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_trace_fit(FitArgs a) {
         P.stop = a.lens_table[3 * sg.lens + 2];
         P.E = P.b1 >= 0 ? L + 2 * (P.b1 - P.b2) : L;
         float loss = 0.0f;
+        uint32_t lost = 0;  // rays with a target that this trace loses
         for (int k = 0; k < a.rpl; k++) {
             const int ray_id = (blk * a.rpl + k) * ADJ_THREADS + lane;
             const bool active = ray_id < R;
@@ -961,8 +967,12 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_trace_fit(FitArgs a) {
                 a.fwd_out[a.ray_total + idx] = alive ? s.pos.y : NAN;
                 a.fwd_out[2 * a.ray_total + idx] = alive ? s.refl : NAN;
             }
+            const float rt = active ? a.tr[idx] : NAN;
+            if (!isnan(rt) &&
+                (!alive || isnan(s.refl) || !isfinite(s.pos.x) || !isfinite(s.pos.y)))
+                lost++;
             if (!alive) continue;
-            const float xt = a.tx[idx], yt = a.ty[idx], rt = a.tr[idx];
+            const float xt = a.tx[idx], yt = a.ty[idx];
             if (isnan(xt) || isnan(yt) || isnan(rt)) continue;
             const float dx = s.pos.x - xt, dy = s.pos.y - yt, dr = (s.refl - rt) * a.inv_refl_scale;
             // a ray reaching the sensor plane almost parallel to it lands at
@@ -984,13 +994,19 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_trace_fit(FitArgs a) {
             }
         }
         const float lv = wave_sum(loss);
-        if (lane == 0) out[4 * a.LF] = lv;
+        const uint32_t lc = wave_sum_u32(lost);
+        if (lane == 0) {
+            out[4 * a.LF] = lv;
+            out[4 * a.LF + 1] = __uint_as_float(lc);  // a count, stored as its bits
+        }
         __syncthreads();  // s_n and s_lacc are rewritten for the next chunk
     }
 }
 
 // Per lens instance: out[i][b LF + e] for b < 3 (radius, centre, coating),
-// out[i][(3 + w) LF + e] (n_eff at wavelength w), out[i][(3 + W) LF] (loss).
+// out[i][(3 + w) LF + e] (n_eff at wavelength w), out[i][(3 + W) LF] (loss),
+// out[i][(3 + W) LF + 1] (rays with a target that the trace loses: counted in
+// 64 bits, written as the nearest float).
 // Instance i owns segments seg_off[i] .. seg_off[i + 1] - 1.  Lane j sums a
 // fixed subset of chunks (or of segments, for n_eff) in order; lanes combine
 // in a fixed tree.
@@ -1001,12 +1017,30 @@ __global__ __launch_bounds__(RED_THREADS) void k_fit_reduce(const float* __restr
                                                             float* __restrict__ out) {
     __shared__ float s_w[RED_THREADS / 64];
     const int t = blockIdx.x, inst = blockIdx.y;
-    const int nout = (3 + W) * LF + 1;
-    const int pstride = 4 * LF + 1;
+    const int nout = (3 + W) * LF + 2;
+    const int pstride = 4 * LF + 2;
     const uint64_t s0 = seg_off[inst], s1 = seg_off[inst + 1];
     float v = 0.0f;
-    if (t < 3 * LF || t == nout - 1) {
-        const int col = t == nout - 1 ? 4 * LF : t;
+    if (t == nout - 1) {
+        // lost-ray counts: integers, summed exactly
+        __shared__ unsigned long long s_n[RED_THREADS / 64];
+        const uint64_t c0 = chunk_offsets[s0], c1 = chunk_offsets[s1];
+        unsigned long long n = 0;
+        for (uint64_t c = c0 + threadIdx.x; c < c1; c += RED_THREADS)
+            n += __float_as_uint(partial[c * pstride + 4 * LF + 1]);
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) n += __shfl_xor(n, m, 64);
+        if ((threadIdx.x & 63) == 0) s_n[threadIdx.x >> 6] = n;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long r = 0;
+            for (int k = 0; k < RED_THREADS / 64; k++) r += s_n[k];
+            out[(long)inst * nout + t] = (float)r;
+        }
+        return;
+    }
+    if (t < 3 * LF || t == nout - 2) {
+        const int col = t == nout - 2 ? 4 * LF : t;
         const uint64_t c0 = chunk_offsets[s0], c1 = chunk_offsets[s1];
         for (uint64_t c = c0 + threadIdx.x; c < c1; c += RED_THREADS) v += partial[c * pstride + col];
     } else {
@@ -1057,7 +1091,7 @@ static FitLayout fit_layout(int max_elements, int checkpoint_every, int workgrou
     F.scratch = o;
     o += align_up((size_t)workgroups * F.slots * ADJ_FIELDS * ADJ_THREADS * sizeof(float), 256);
     F.partial = o;
-    o += align_up((size_t)chunk_count * (4 * max_elements + 1) * sizeof(float), 256);
+    o += align_up((size_t)chunk_count * (4 * max_elements + 2) * sizeof(float), 256);
     F.total = o;
     return F;
 }
@@ -1261,7 +1295,7 @@ extern "C" int rf_trace_fit(const rf_lens_element* lenses, const float* coatings
     hipLaunchKernelGGL(k_queue_reset_fit, dim3(1), dim3(1), 0, s, a.head);
     const size_t lds = (size_t)ADJ_LDS_ROWS * max_elements * ADJ_THREADS * sizeof(float);
     hipLaunchKernelGGL(kern, dim3(workgroups), dim3(ADJ_THREADS), lds, s, a);
-    const int nout = (3 + wavelength_count) * max_elements + 1;
+    const int nout = (3 + wavelength_count) * max_elements + 2;
     hipLaunchKernelGGL(k_fit_reduce, dim3(nout, lens_instances), dim3(RED_THREADS), 0, s, a.partial,
                        chunk_offsets, lens_segment_offsets, max_elements, wavelength_count,
                        grad_out);

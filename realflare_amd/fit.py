@@ -118,7 +118,7 @@ class FitBatch:
         if ws == 0:
             raise ValueError('rf_trace_fit_workspace_size rejected the batch')
         self.ws = torch.empty((ws,), dtype=torch.uint8, device=dev)
-        self.nout = (3 + self.W) * self.LF + 1
+        self.nout = (3 + self.W) * self.LF + 2
         self.grad = torch.empty((self.I, self.nout), dtype=torch.float32, device=dev)
         self.fwd = torch.empty((3, self.rays), dtype=torch.float32, device=dev)
         nan = torch.full((self.rays,), float('nan'), dtype=torch.float32, device=dev)
@@ -230,7 +230,7 @@ class FitBatch:
         self.upload(params)
         self.launch()
         g = self.grad.double().cpu().numpy()
-        lost = self.lost_rays() if self.lost_penalty > 0 else [0] * self.I
+        # the launch counts each instance's lost rays (row[(3 + W) LF + 1])
         losses, grads = [], []
         LF, W = self.LF, self.W
         for k, (it, p) in enumerate(zip(self.inst, params)):
@@ -238,7 +238,8 @@ class FitBatch:
             row = g[k]
             compact = np.concatenate([row[b * LF:b * LF + L] for b in range(3 + W)])
             grads.append(chain_grads(it.static, self.wavelengths, p, compact))
-            losses.append(row[(3 + W) * LF] + self.loss_scale * self.lost_penalty * lost[k])
+            lost = row[(3 + W) * LF + 1]
+            losses.append(row[(3 + W) * LF] + self.loss_scale * self.lost_penalty * lost)
         return np.array(losses), grads
 
 

@@ -192,3 +192,27 @@ def test_fit_100_steps_with_fp64_error_per_step():
     assert hist[-1]['best_loss'] <= 0.1 * hist[0]['loss']
     assert max(h['fp64_pos_err_median'] for h in hist) <= 1e-5
     assert max(h['fp64_pos_err_p99'] for h in hist) <= 1e-3
+
+
+def test_fit_counts_lost_rays_in_the_launch():
+    """The launch's per-instance count of rays that have a target but that the
+    trace loses equals the count over its forward outputs (NaN reflectance or
+    a non-finite position where the target's reflectance is defined), and the
+    reported loss adds lost_penalty per such ray.  The lenses are evaluated
+    30 % away from the ones the targets come from, so that rays cross
+    visibility edges."""
+    from realflare_amd.adjoint import perturbed
+
+    inst = _instances(grid=16)
+    fb = FitBatch(inst, WL, RES, SENSOR, checkpoint_every=4, rays_per_lane=2, workgroups=64)
+    fb.set_targets([i.params for i in inst], image_window=False)
+    losses, _ = fb.step([perturbed(i.params, i.static, 0.3, seed=3 + k) for k, i in enumerate(inst)])
+    torch.cuda.synchronize()
+    counted = fb.grad[:, (3 + fb.W) * fb.LF + 1].cpu().numpy()
+    expect = fb.lost_rays()
+    print(f'lost rays: launch {counted.tolist()}, forward outputs {expect}', flush=True)
+    assert sum(expect) > 0
+    np.testing.assert_array_equal(counted, np.array(expect, np.float32))
+    base = fb.grad[:, (3 + fb.W) * fb.LF].double().cpu().numpy()
+    np.testing.assert_allclose(losses, base + fb.loss_scale * fb.lost_penalty * np.array(expect),
+                               rtol=1e-12)

@@ -17,7 +17,7 @@ def test_library_exports_header_symbols():
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
     lib.rf_abi_version.restype = ctypes.c_int
-    assert lib.rf_abi_version() == 2
+    assert lib.rf_abi_version() == 3
 
 
 def test_invalid_arguments_report_error_without_gpu():
