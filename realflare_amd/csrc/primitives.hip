@@ -48,7 +48,14 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const uint32_t* __
     const uint64_t n = n_ptr ? *n_ptr : n_fixed;
     const uint64_t base = (uint64_t)blockIdx.x * SCAN_CHUNK;
     uint64_t s = 0;
-    if (base < n) {
+    if (base + SCAN_CHUNK <= n) {
+        // whole chunk: 16-byte loads, each wave covering 1 KiB per step
+#pragma unroll
+        for (int i = 0; i < SCAN_ITEMS / 4; i++) {
+            const uint4 q = reinterpret_cast<const uint4*>(in + base)[i * SCAN_THREADS + threadIdx.x];
+            s += (uint64_t)q.x + q.y + q.z + q.w;
+        }
+    } else if (base < n) {
         for (int i = 0; i < SCAN_ITEMS; i++) {
             uint64_t idx = base + (uint64_t)i * SCAN_THREADS + threadIdx.x;
             if (idx < n) s += in[idx];
@@ -63,13 +70,25 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const uint32_t* __
 // total to *total_out.
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_sums(uint64_t* __restrict__ sums, int nblocks,
                                                              uint64_t* __restrict__ total_out) {
+    // each thread scans SCAN_ITEMS consecutive sums serially, then the
+    // block scans the per-thread totals: one pass per SCAN_CHUNK sums
     uint64_t carry = 0;
-    for (int start = 0; start < nblocks; start += SCAN_THREADS) {
-        int i = start + threadIdx.x;
-        uint64_t v = i < nblocks ? sums[i] : 0;
+    for (int start = 0; start < nblocks; start += SCAN_CHUNK) {
+        const int first = start + threadIdx.x * SCAN_ITEMS;
+        uint64_t v[SCAN_ITEMS];
+        uint64_t t = 0;
+#pragma unroll
+        for (int i = 0; i < SCAN_ITEMS; i++) {
+            v[i] = first + i < nblocks ? sums[first + i] : 0;
+            t += v[i];
+        }
         uint64_t total;
-        uint64_t ex = block_excl_scan_u64(v, &total);
-        if (i < nblocks) sums[i] = carry + ex;
+        uint64_t run = carry + block_excl_scan_u64(t, &total);
+#pragma unroll
+        for (int i = 0; i < SCAN_ITEMS; i++) {
+            if (first + i < nblocks) sums[first + i] = run;
+            run += v[i];
+        }
         carry += total;
     }
     if (threadIdx.x == 0 && total_out) *total_out = carry;
@@ -82,23 +101,49 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const uint32_t* __r
     const uint64_t n = n_ptr ? *n_ptr : n_fixed;
     const uint64_t base = (uint64_t)blockIdx.x * SCAN_CHUNK;
     if (base >= n) return;
-    // each thread owns SCAN_ITEMS consecutive elements
+    // each thread owns SCAN_ITEMS consecutive elements, read and written as
+    // 16-byte vectors where the whole run lies below n (both arrays are
+    // 16-byte aligned, and so is every run: SCAN_ITEMS is a multiple of 4)
     const uint64_t first = base + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    const bool whole = first + SCAN_ITEMS <= n;
     uint32_t v[SCAN_ITEMS];
     uint64_t s = 0;
+    if (whole) {
 #pragma unroll
-    for (int i = 0; i < SCAN_ITEMS; i++) {
-        uint64_t idx = first + i;
-        v[i] = idx < n ? in[idx] : 0u;
-        s += v[i];
+        for (int i = 0; i < SCAN_ITEMS / 4; i++) {
+            const uint4 q = reinterpret_cast<const uint4*>(in + first)[i];
+            v[4 * i] = q.x;
+            v[4 * i + 1] = q.y;
+            v[4 * i + 2] = q.z;
+            v[4 * i + 3] = q.w;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < SCAN_ITEMS; i++) {
+            const uint64_t idx = first + i;
+            v[i] = idx < n ? in[idx] : 0u;
+        }
     }
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) s += v[i];
     uint64_t total;
     uint64_t run = block_offs[blockIdx.x] + block_excl_scan_u64(s, &total);
+    if (whole) {
 #pragma unroll
-    for (int i = 0; i < SCAN_ITEMS; i++) {
-        uint64_t idx = first + i;
-        if (idx < n) out[idx] = run;
-        run += v[i];
+        for (int i = 0; i < SCAN_ITEMS / 2; i++) {
+            const uint64_t a = run;
+            run += v[2 * i];
+            const uint64_t b = run;
+            run += v[2 * i + 1];
+            reinterpret_cast<ulonglong2*>(out + first)[i] = make_ulonglong2(a, b);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < SCAN_ITEMS; i++) {
+            const uint64_t idx = first + i;
+            if (idx < n) out[idx] = run;
+            run += v[i];
+        }
     }
 }
 
@@ -109,6 +154,8 @@ size_t scan_workspace_bytes(uint64_t capacity) {
 
 int scan_exclusive_u32(const uint32_t* in, uint64_t* out, const uint64_t* n_dev, uint64_t capacity,
                        uint64_t* total_dev, void* ws, hipStream_t s) {
+    if (((uintptr_t)in | (uintptr_t)out) & 15)
+        return set_error(RF_ERR_INVALID, "scan_exclusive_u32: arrays must be 16-byte aligned");
     int nb = (int)((capacity + SCAN_CHUNK - 1) / SCAN_CHUNK);
     if (nb == 0) nb = 1;
     uint64_t* sums = (uint64_t*)ws;
