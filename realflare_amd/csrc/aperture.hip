@@ -17,12 +17,6 @@ namespace rf {
 
 static constexpr int AP_MAX_SHAPES = 1000;
 
-__device__ __forceinline__ float noise3(float x, float y, float z) {
-#pragma clang fp contract(off)
-    float v = sinf(x * 112.9898f + y * 179.233f + z * 237.212f) * 43758.5453f;
-    return fminf(v - floorf(v), 0x1.fffffep-1f);
-}
-
 __device__ __forceinline__ float wrap1(float v) { return fmodf(1.0f + fmodf(v, 1.0f), 1.0f); }
 
 __device__ __forceinline__ float rectangle_sdf(float px, float py, float hx, float hy) {

@@ -126,6 +126,75 @@ __device__ __forceinline__ float sample_bilinear(const float* __restrict__ tex, 
            a * b * T[3];
 }
 
+// sin of a float argument, evaluated in double: k = the integer nearest
+// x / pi, r = x - k pi with pi split into two doubles (|r| <= pi/2 up to the
+// rounding of k, with an error far below a float ulp of the result for
+// |x| < 2^31), sin x = (-1)^k sin r, and sin r from its Taylor series through
+// the r^17 term (the truncation is below 5e-14 at |r| = pi/2), rounded once
+// to float: the correctly rounded sine except within ~1e-13 of a rounding
+// midpoint.  The noise's arguments reach 10^6, where a single-precision
+// sine would take its long reduction path.
+__device__ __forceinline__ float sin_reduced(float x) {
+    const double kPiHi = 3.141592653589793116;    // double nearest pi
+    const double kPiLo = 1.2246467991473532e-16;  // pi - kPiHi
+    const double xd = (double)x;
+    const double k = rint(xd * 0.31830988618379067154);  // 1/pi
+    const double r = fma(-k, kPiLo, fma(-k, kPiHi, xd));
+    const double s = r * r;
+    // (-1)^n / (2n + 1)! for n = 8 down to 1
+    double p = 1.0 / 355687428096000.0;
+    p = fma(p, s, -1.0 / 1307674368000.0);
+    p = fma(p, s, 1.0 / 6227020800.0);
+    p = fma(p, s, -1.0 / 39916800.0);
+    p = fma(p, s, 1.0 / 362880.0);
+    p = fma(p, s, -1.0 / 5040.0);
+    p = fma(p, s, 1.0 / 120.0);
+    p = fma(p, s, -1.0 / 6.0);
+    const float v = (float)fma(r * s, p, r);
+    return ((int)k & 1) ? -v : v;
+}
+
+// sin and cos of a float argument the same way, with the reduction by pi/2
+// (|r| <= pi/4; Taylor series through r^13 for the sine and r^14 for the
+// cosine, truncations below 3e-14) and the quadrant k mod 4 choosing
+// +-sin r or +-cos r for each.
+__device__ __forceinline__ void sincos_reduced(float x, float* sn, float* cs) {
+    const double kPio2Hi = 1.5707963267948966;     // double nearest pi/2
+    const double kPio2Lo = 6.123233995736766e-17;  // pi/2 - kPio2Hi
+    const double xd = (double)x;
+    const double k = rint(xd * 0.63661977236758134);  // 2/pi
+    const double r = fma(-k, kPio2Lo, fma(-k, kPio2Hi, xd));
+    const double s = r * r;
+    // (-1)^n / (2n + 1)! for n = 6 down to 1, (-1)^(n+1) / (2n)! for n = 7 down to 1
+    double ps = 1.0 / 6227020800.0;
+    ps = fma(ps, s, -1.0 / 39916800.0);
+    ps = fma(ps, s, 1.0 / 362880.0);
+    ps = fma(ps, s, -1.0 / 5040.0);
+    ps = fma(ps, s, 1.0 / 120.0);
+    ps = fma(ps, s, -1.0 / 6.0);
+    double pc = 1.0 / 87178291200.0;
+    pc = fma(pc, s, -1.0 / 479001600.0);
+    pc = fma(pc, s, 1.0 / 3628800.0);
+    pc = fma(pc, s, -1.0 / 40320.0);
+    pc = fma(pc, s, 1.0 / 720.0);
+    pc = fma(pc, s, -1.0 / 24.0);
+    pc = fma(pc, s, 0.5);
+    const float sr = (float)fma(r * s, ps, r);
+    const float cr = (float)fma(-s, pc, 1.0);
+    const int q = (int)k;
+    const float a = (q & 1) ? cr : sr, b = (q & 1) ? sr : cr;
+    *sn = (q & 2) ? -a : a;
+    *cs = ((q + 1) & 2) ? -b : b;
+}
+
+// noise (noise.cl:1-5): fract(sin(x 112.9898 + y 179.233 + z 237.212) 43758.5453),
+// with the sine rounded correctly (its error is multiplied by 43758)
+__device__ __forceinline__ float noise3(float x, float y, float z) {
+#pragma clang fp contract(off)
+    float v = sin_reduced(x * 112.9898f + y * 179.233f + z * 237.212f) * 43758.5453f;
+    return fminf(v - floorf(v), 0x1.fffffep-1f);
+}
+
 // color.cl:3-10, with the matrix applied in double as the source's double
 // literals ask for.
 __device__ __forceinline__ void xyz_to_ap1(float x, float y, float z, float* o) {

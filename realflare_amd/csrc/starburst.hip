@@ -16,27 +16,6 @@ namespace rf {
 
 static constexpr int SB_MAX_CACHED = 1024;
 
-// sin of a float argument, reduced in double (Cody-Waite with a two-part
-// pi/2): k = the integer nearest x 2/pi; r = (x - k h) - k l with
-// h = double(pi/2), l = pi/2 - h, the first step exact (x and k h are
-// multiples of 2^-52 and the difference is below 1 in magnitude for
-// |x| < 2^31); then sin x = +-sin r or +-cos r by the quadrant k mod 4, with
-// |r| <= pi/4 handled by the single-precision sine and cosine.  The noise's
-// arguments reach 10^6, where the single-precision sine takes its long
-// reduction path; this one costs a handful of double operations instead.
-__device__ __forceinline__ float sin_reduced(float x) {
-    const double kPio2Hi = 1.5707963267948966;     // double nearest pi/2
-    const double kPio2Lo = 6.123233995736766e-17;  // pi/2 - kPio2Hi
-    const double xd = (double)x;
-    const double k = rint(xd * 0.63661977236758134);  // 2/pi
-    const double r = fma(-k, kPio2Lo, fma(-k, kPio2Hi, xd));
-    float sr, cr;
-    sincosf((float)r, &sr, &cr);
-    const int q = (int)(k - 4.0 * floor(k * 0.25));
-    const float v = (q & 1) ? cr : sr;
-    return (q & 2) ? -v : v;
-}
-
 // x / b correctly rounded, from ib = 1 / b correctly rounded (Markstein's
 // correction: q = x ib is within an ulp of x / b, the remainder x - q b is
 // exact with a fused multiply-add, and q + (x - q b) ib rounded once is the
@@ -49,34 +28,31 @@ __device__ __forceinline__ float div_rn(float x, float b, float ib) {
     return __builtin_fmaf(r, ib, q);
 }
 
-// noise (noise.cl:1-5): fract(sin(x 112.9898 + y 179.233 + z 237.212) 43758.5453)
-__device__ __forceinline__ float sb_noise(float x, float y, float z) {
-#pragma clang fp contract(off)
-    float v = sin_reduced(x * 112.9898f + y * 179.233f + z * 237.212f) * 43758.5453f;
-    return fminf(v - floorf(v), 0x1.fffffep-1f);
-}
 
 __global__ __launch_bounds__(256) void k_starburst(const float* __restrict__ spectrum, int sw, int sh,
                                                    const float* __restrict__ light, int light_n,
                                                    rf_starburst_params p, float4* __restrict__ out) {
 #pragma clang fp contract(off)
-    __shared__ float4 s_xyz[SB_MAX_CACHED];
-    __shared__ float2 s_sc[SB_MAX_CACHED];  // per sample: wavelength scale, its reciprocal
+    // per sample: the X, Y, Z weights and the wavelength scale, and the
+    // scale's reciprocal (20 KiB at the cap, so eight workgroups fit a CU;
+    // the table's fourth component never reaches the output)
+    __shared__ float4 s_xyzs[SB_MAX_CACHED];
+    __shared__ float s_isc[SB_MAX_CACHED];
     const float pi = 3.14159265358979323846f;
     const int samples = p.samples;
     const bool cached = samples <= SB_MAX_CACHED;
     if (cached) {
         for (int t = threadIdx.x; t < samples; t += blockDim.x) {
             const float step = (float)t / (float)samples;
+            const float wavelength = step * (float)(730 - 390) + 390.0f;
+            const float sc = fmaxf(1 - (1 - (wavelength / 560.0f)) * 1.0f, FLT_MIN);
             float4 v;
             v.x = sample_bilinear(light + 0, light_n, 1, 4, step, 0.0f, false);
             v.y = sample_bilinear(light + 1, light_n, 1, 4, step, 0.0f, false);
             v.z = sample_bilinear(light + 2, light_n, 1, 4, step, 0.0f, false);
-            v.w = sample_bilinear(light + 3, light_n, 1, 4, step, 0.0f, false);
-            s_xyz[t] = v;
-            const float wavelength = step * (float)(730 - 390) + 390.0f;
-            const float sc = fmaxf(1 - (1 - (wavelength / 560.0f)) * 1.0f, FLT_MIN);
-            s_sc[t] = make_float2(sc, 1.0f / sc);
+            v.w = sc;
+            s_xyzs[t] = v;
+            s_isc[t] = 1.0f / sc;
         }
         __syncthreads();
     }
@@ -91,7 +67,7 @@ __global__ __launch_bounds__(256) void k_starburst(const float* __restrict__ spe
     nx = nx / fmaxf(p.size[0], FLT_MIN);
     ny = ny / fmaxf(p.size[1], FLT_MIN);
     const int lambda_delta = 730 - 390;
-    float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f, r3 = 0.0f;
+    float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f;
     const float fx = (float)px, fy = (float)py;
     // With rotation 0 (the default, and a weight >= 0, so that the power
     // below is finite) the rotation angle is exactly +-0: cos = 1 and
@@ -105,62 +81,57 @@ __global__ __launch_bounds__(256) void k_starburst(const float* __restrict__ spe
     const bool rotate = !(p.rotation == 0.0f && p.rotation_weight >= 0.0f);
     for (int t = 0; t < samples; ++t) {
         const float step = (float)t / (float)samples;
-        float sc, isc;
+        float4 xyzs;  // X, Y, Z weights and the wavelength scale
+        float isc;
         if (cached) {
-            const float2 c = s_sc[t];
-            sc = c.x;
-            isc = c.y;
+            xyzs = s_xyzs[t];
+            isc = s_isc[t];
         } else {
             const float wavelength = step * (float)lambda_delta + 390.0f;
-            sc = fmaxf(1 - (1 - (wavelength / 560.0f)) * 1.0f, FLT_MIN);
-            isc = 1.0f / sc;
+            xyzs.w = fmaxf(1 - (1 - (wavelength / 560.0f)) * 1.0f, FLT_MIN);
+            isc = 1.0f / xyzs.w;
+            xyzs.x = sample_bilinear(light + 0, light_n, 1, 4, step, 0.0f, false);
+            xyzs.y = sample_bilinear(light + 1, light_n, 1, 4, step, 0.0f, false);
+            xyzs.z = sample_bilinear(light + 2, light_n, 1, 4, step, 0.0f, false);
         }
+        const float sc = xyzs.w;
         const float seed = (float)(t * 4);
         float x = nx, y = ny;
         if (jitter) {
-            const float radius = p.blur * sqrtf(sb_noise(fx, fy, seed));
-            const float alpha = sb_noise(fx, fy, seed + 1) * 2 * pi;
-            x += cosf(alpha) * radius;
-            y += sinf(alpha) * radius;
+            const float radius = p.blur * sqrtf(noise3(fx, fy, seed));
+            const float alpha = noise3(fx, fy, seed + 1) * 2 * pi;
+            float sa, ca;
+            sincos_reduced(alpha, &sa, &ca);
+            x += ca * radius;
+            y += sa * radius;
         }
         x = div_rn(x, sc, isc);
         y = div_rn(y, sc, isc);
         float rx = x, ry = y;
         if (rotate) {
-            const float sign = (sb_noise(fx, fy, seed + 2) > 0.5f) ? 1.0f : -1.0f;
+            const float sign = (noise3(fx, fy, seed + 2) > 0.5f) ? 1.0f : -1.0f;
             const float angle =
-                sign * p.rotation * (1.0f - powf(sb_noise(fx, fy, seed + 3), p.rotation_weight));
-            const float sa = sinf(angle), ca = cosf(angle);
+                sign * p.rotation * (1.0f - powf(noise3(fx, fy, seed + 3), p.rotation_weight));
+            float sa, ca;
+            sincos_reduced(angle, &sa, &ca);
             rx = x * ca + y * sa;
             ry = y * ca - x * sa;
         }
         const float u = rx * 0.5f + 0.5f, v = ry * 0.5f + 0.5f;
         const float fi = sample_bilinear(spectrum, sw, sh, 1, u, v, false);
-        float4 xyz;
-        if (cached) {
-            xyz = s_xyz[t];
-        } else {
-            xyz.x = sample_bilinear(light + 0, light_n, 1, 4, step, 0.0f, false);
-            xyz.y = sample_bilinear(light + 1, light_n, 1, 4, step, 0.0f, false);
-            xyz.z = sample_bilinear(light + 2, light_n, 1, 4, step, 0.0f, false);
-            xyz.w = sample_bilinear(light + 3, light_n, 1, 4, step, 0.0f, false);
-        }
-        r0 += xyz.x * fi;
-        r1 += xyz.y * fi;
-        r2 += xyz.z * fi;
-        r3 += xyz.w * fi;
+        r0 += xyzs.x * fi;
+        r1 += xyzs.y * fi;
+        r2 += xyzs.z * fi;
     }
     r0 /= (float)samples;
     r1 /= (float)samples;
     r2 /= (float)samples;
-    r3 /= (float)samples;
     if (!isnan(p.vignetting[0])) {
         const float sdf = sqrtf(nx * nx + ny * ny) - 0.0f;
         const float f = smoothstepf(p.vignetting[1], p.vignetting[0], sdf);
         r0 *= f;
         r1 *= f;
         r2 *= f;
-        r3 *= f;
     }
     r0 *= p.intensity;
     r1 *= p.intensity;

@@ -4,14 +4,16 @@ FLARE_STARBURST frame.
 
 Bars: aperture images with soft edges within 1e-5 absolute, hard edges may
 flip pixels lying on the edge (<= 0.1% of pixels); scratch and dust masks
-within 5e-2 relative L1, because their centres come from
+within 1e-2 relative L1, because their centres come from
 fract(sin(x) * 43758.5453) at arguments up to ~3e5, where one ulp of sin
-moves a centre by about half a pixel; FFT within 1e-5 of the
+moves a centre by about half a pixel: the kernels round that sine
+correctly, the oracle takes the C library's sinf, which is off by one ulp
+for about 1.3% of such arguments; FFT within 1e-5 of the
 norm; power spectrum and ghost texture (complex64 FFTs against numpy's
 complex128) within 1e-4 relative L1; starburst within 1e-5 relative L1 with
-the reference defaults (no blur/rotation jitter) and 1e-2 with jitter, whose
-hash noise amplifies ulp-level sin differences of large arguments; a whole
-FLARE_STARBURST frame within 1e-3 relative L1.
+the reference defaults (no blur/rotation jitter) and 1e-3 with jitter, whose
+hash noise meets the same sine differences; a whole FLARE_STARBURST frame
+within 1e-3 relative L1.
 """
 import ctypes
 
@@ -80,7 +82,7 @@ def test_aperture_matches_oracle(kind, oracle_lib):
         # up to ~3e5: one ulp of sin moves a centre by ~1e-3 of the aperture
         err = rel_l1(img, ref)
         print(f'noise masks rel L1 {err:.2e}, max {diff.max():.3f}', flush=True)
-        assert err <= 5e-2
+        assert err <= 1e-2
     else:
         assert diff.max() <= 1e-5, diff.max()
 
@@ -219,7 +221,7 @@ def test_starburst_matches_oracle(jitter, oracle_lib):
     err = rel_l1(out.cpu().numpy(), ref)
     print(f'starburst jitter={jitter} rel L1 {err:.2e}', flush=True)
     assert np.abs(ref).sum() > 0
-    assert err <= (1e-2 if jitter else 1e-5)
+    assert err <= (1e-3 if jitter else 1e-5)
 
 
 @pytest.mark.parametrize('blur', [0.0, 1.0])
