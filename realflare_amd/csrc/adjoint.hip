@@ -24,6 +24,7 @@
 // deterministic and independent of scheduling.
 #include <cmath>
 
+#include "fresnel.h"
 #include "primitives.h"
 
 namespace rf {
@@ -39,38 +40,12 @@ struct St {
 // --- forward pieces (same arithmetic as trace.hip) ---------------------------
 
 __device__ __noinline__ float fresnel_f(float theta0, float n1, float n2) {
-#pragma clang fp contract(off)
-    float theta1 = asinf(sinf(theta0) * n1 / n2);
-    float cosi = cosf(theta0);
-    float cost = cosf(theta1);
-    float rs = ((n1 * cosi) - (n2 * cost)) / ((n1 * cosi) + (n2 * cost));
-    float rp = ((n1 * cost) - (n2 * cosi)) / ((n1 * cost) + (n2 * cosi));
-    return (rs * rs + rp * rp) / 2;
+    return fresnel_terms(theta0, n1, n2).r;
 }
 
-__device__ __noinline__ float fresnel_ar_f(float theta0, float lambda, float d1, float n0, float n1, float n2) {
-#pragma clang fp contract(off)
-    const float pi = 3.14159265358979323846f;
-    float s0 = sinf(theta0);
-    float theta1 = asinf(s0 * n0 / n1);
-    float theta2 = asinf(s0 * n0 / n2);
-    float sp01 = sinf(theta0 + theta1);
-    float rs01 = -sinf(theta0 - theta1) / sp01;
-    float rp01 = tanf(theta0 - theta1) / tanf(theta0 + theta1);
-    float ts01 = 2 * sinf(theta1) * cosf(theta0) / sp01;
-    float tp01 = ts01 * cosf(theta0 - theta1);
-    float rs12 = -sinf(theta1 - theta2) / sinf(theta1 + theta2);
-    float rp12 = tanf(theta1 - theta2) / tanf(theta1 + theta2);
-    float ris = ts01 * ts01 * rs12;
-    float rip = tp01 * tp01 * rp12;
-    float dy = d1 * n1;
-    float dx = tanf(theta1) * dy;
-    float delay = sqrtf(dx * dx + dy * dy);
-    float rel_phase = 4 * pi / lambda * (delay - dx * s0);
-    float c = cosf(rel_phase);
-    float out_s2 = rs01 * rs01 + ris * ris + 2 * rs01 * ris * c;
-    float out_p2 = rp01 * rp01 + rip * rip + 2 * rp01 * rip * c;
-    return (out_s2 + out_p2) / 2;
+__device__ __noinline__ float fresnel_ar_f(float theta0, float lambda, float d1, float n0, float n1,
+                                           float n2) {
+    return fresnel_ar_terms(theta0, lambda, d1, n0, n1, n2).r;
 }
 
 __device__ float dispersion_f(float lambda, const float* c) {
@@ -248,22 +223,10 @@ struct FresnelGrad {
 // fresnel (raytracing.cl:1-16): R and Rb * dR/d(theta0, n1, n2), for R > 0
 __device__ __noinline__ FresnelGrad fresnel_grad(float theta0, float n1, float n2, float Rb) {
     FresnelGrad g{0, 0, 0, 0, 0, 0};
-    float s0, c0, a1, th1, c1, A, B, Cc, D, rs, rp;
-    {
-#pragma clang fp contract(off)
-        s0 = sinf(theta0);
-        a1 = s0 * n1 / n2;
-        th1 = asinf(a1);
-        c0 = cosf(theta0);
-        c1 = cosf(th1);
-        A = n1 * c0;
-        B = n2 * c1;
-        Cc = n1 * c1;
-        D = n2 * c0;
-        rs = (A - B) / (A + B);
-        rp = (Cc - D) / (Cc + D);
-        g.r = (rs * rs + rp * rp) / 2;
-    }
+    const FresnelTerms T = fresnel_terms(theta0, n1, n2);
+    const float s0 = T.s0, c0 = T.c0, a1 = T.a1, c1 = T.c1, A = T.A, B = T.B, Cc = T.Cc, D = T.D;
+    const float rs = T.rs, rp = T.rp;
+    g.r = T.r;
     if (!(g.r > 0)) return g;
     const float rsb = Rb * rs, rpb = Rb * rp;
     // rs = (A - B)/(A + B): d/dA = 2B/(A+B)^2, d/dB = -2A/(A+B)^2
@@ -274,7 +237,7 @@ __device__ __noinline__ FresnelGrad fresnel_grad(float theta0, float n1, float n
     g.n2 = Bb * c1 + Db * c0;
     const float c0b = Ab * n1 + Db * n2;
     const float c1b = Bb * n2 + Cb2 * n1;
-    const float th1b = -c1b * a1;  // sin(asin a1) = a1
+    const float th1b = -c1b * a1;  // d cos(theta1) / d theta1 = -sin theta1 = -a1
     float s0b = 0.0f;
     const float q = 1 - a1 * a1;
     if (q > 1e-12f) {
@@ -291,50 +254,15 @@ __device__ __noinline__ FresnelGrad fresnel_grad(float theta0, float n1, float n
 __device__ __noinline__ FresnelGrad fresnel_ar_grad(float theta0, float lambda, float d1, float n0,
                                                     float n1, float n2, float Rb) {
     FresnelGrad g{0, 0, 0, 0, 0, 0};
-    const float pi = 3.14159265358979323846f;
-    float s0, c0, a1, a2, th1, th2, dm, dp, sdm, sdp, tdm, tdp, cdm, s1, ts01, tp01, em, ep, sem,
-        sep, tem, tep, rs01, rp01, rs12, rp12, ris, rip, dy, t1, dx, delay, k4, ph, cph;
-    {
-#pragma clang fp contract(off)
-        s0 = sinf(theta0);
-        a1 = s0 * n0 / n1;
-        a2 = s0 * n0 / n2;
-        th1 = asinf(a1);
-        th2 = asinf(a2);
-        dm = theta0 - th1;
-        dp = theta0 + th1;
-        sdp = sinf(dp);
-        sdm = sinf(dm);
-        rs01 = -sdm / sdp;
-        tdm = tanf(dm);
-        tdp = tanf(dp);
-        rp01 = tdm / tdp;
-        s1 = sinf(th1);
-        c0 = cosf(theta0);
-        ts01 = 2 * s1 * c0 / sdp;
-        cdm = cosf(dm);
-        tp01 = ts01 * cdm;
-        em = th1 - th2;
-        ep = th1 + th2;
-        sem = sinf(em);
-        sep = sinf(ep);
-        rs12 = -sem / sep;
-        tem = tanf(em);
-        tep = tanf(ep);
-        rp12 = tem / tep;
-        ris = ts01 * ts01 * rs12;
-        rip = tp01 * tp01 * rp12;
-        dy = d1 * n1;
-        t1 = tanf(th1);
-        dx = t1 * dy;
-        delay = sqrtf(dx * dx + dy * dy);
-        k4 = 4 * pi / lambda;
-        ph = k4 * (delay - dx * s0);
-        cph = cosf(ph);
-        const float out_s2 = rs01 * rs01 + ris * ris + 2 * rs01 * ris * cph;
-        const float out_p2 = rp01 * rp01 + rip * rip + 2 * rp01 * rip * cph;
-        g.r = (out_s2 + out_p2) / 2;
-    }
+    const FresnelArTerms T = fresnel_ar_terms(theta0, lambda, d1, n0, n1, n2);
+    const float s0 = T.s0, c0 = T.c0, a1 = T.a1, a2 = T.a2, c1 = T.c1, c2 = T.c2;
+    const float sdm = T.sdm, sdp = T.sdp, cdm = T.cdm, cdp = T.cdp, tdm = T.tdm, tdp = T.tdp;
+    const float sem = T.sem, sep = T.sep, cem = T.cem, cep = T.cep, tem = T.tem, tep = T.tep;
+    const float rs01 = T.rs01, rp01 = T.rp01, ts01 = T.ts01, tp01 = T.tp01, rs12 = T.rs12,
+                rp12 = T.rp12, ris = T.ris, rip = T.rip;
+    const float dy = T.dy, t1 = T.t1, dx = T.dx, delay = T.delay, k4 = T.k4, ph = T.ph, cph = T.cph;
+    const float s1 = a1;  // sin theta1
+    g.r = T.r;
     if (!(g.r > 0)) return g;
     const float osb = Rb * 0.5f, opb = Rb * 0.5f;
     const float rs01b = osb * (2 * rs01 + 2 * ris * cph);
@@ -364,10 +292,7 @@ __device__ __noinline__ FresnelGrad fresnel_ar_grad(float theta0, float lambda, 
     const float itep = frcp(tep), isep = frcp(sep);
     const float temb = rp12b * itep, tepb = -rp12b * tem * itep * itep;
     const float semb = -rs12b * isep, sepb = rs12b * sem * isep * isep;
-    // th1, th2 lie in [-pi/2, pi/2]: cos th = sqrt(1 - a^2), sin th = a
     const float q1 = 1 - a1 * a1, q2 = 1 - a2 * a2;
-    const float c1 = fsqrt(fmaxf(q1, 0.0f)), c2 = fsqrt(fmaxf(q2, 0.0f));
-    const float cem = c1 * c2 + a1 * a2, cep = c1 * c2 - a1 * a2;
     const float emb = temb * (1 + tem * tem) + semb * cem;
     const float epb = tepb * (1 + tep * tep) + sepb * cep;
     th1b += emb + epb;
@@ -383,7 +308,6 @@ __device__ __noinline__ FresnelGrad fresnel_ar_grad(float theta0, float lambda, 
     const float tdmb = rp01b * itdp, tdpb = -rp01b * tdm * itdp * itdp;
     const float sdmb = -rs01b * isdp;
     sdpb += rs01b * sdm * isdp * isdp;
-    const float cdp = c0 * c1 - s0 * a1;  // cos(theta0 + th1)
     dmb += tdmb * (1 + tdm * tdm) + sdmb * cdm;
     const float dpb = tdpb * (1 + tdp * tdp) + sdpb * cdp;
     float t0 = dmb + dpb;

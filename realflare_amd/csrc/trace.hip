@@ -10,47 +10,20 @@
 // raytracing.cl:107-120) is evaluated once per workgroup into LDS.
 #include <cmath>
 
+#include "fresnel.h"
 #include "rf_common.h"
 
 namespace rf {
 
 enum { OUT_AOS = 0, OUT_PLANES = 1, OUT_INTERSECTIONS = 2 };
 
-// raytracing.cl:1-16
+// raytracing.cl:1-16 and 18-59, in the angle-sum form of fresnel.h
 __device__ float fresnel(float theta0, float n1, float n2) {
-#pragma clang fp contract(off)
-    float theta1 = asinf(sinf(theta0) * n1 / n2);
-    float cosi = cosf(theta0);
-    float cost = cosf(theta1);
-    float rs = ((n1 * cosi) - (n2 * cost)) / ((n1 * cosi) + (n2 * cost));
-    float rp = ((n1 * cost) - (n2 * cosi)) / ((n1 * cost) + (n2 * cosi));
-    return (rs * rs + rp * rp) / 2;
+    return fresnel_terms(theta0, n1, n2).r;
 }
 
-// raytracing.cl:18-59 (single-layer anti-reflective coating, Ritschel 2009)
 __device__ float fresnel_ar(float theta0, float lambda, float d1, float n0, float n1, float n2) {
-#pragma clang fp contract(off)
-    const float pi = 3.14159265358979323846f;
-    float s0 = sinf(theta0);
-    float theta1 = asinf(s0 * n0 / n1);
-    float theta2 = asinf(s0 * n0 / n2);
-    float sp01 = sinf(theta0 + theta1);
-    float rs01 = -sinf(theta0 - theta1) / sp01;
-    float rp01 = tanf(theta0 - theta1) / tanf(theta0 + theta1);
-    float ts01 = 2 * sinf(theta1) * cosf(theta0) / sp01;
-    float tp01 = ts01 * cosf(theta0 - theta1);
-    float rs12 = -sinf(theta1 - theta2) / sinf(theta1 + theta2);
-    float rp12 = tanf(theta1 - theta2) / tanf(theta1 + theta2);
-    float ris = ts01 * ts01 * rs12;
-    float rip = tp01 * tp01 * rp12;
-    float dy = d1 * n1;
-    float dx = tanf(theta1) * dy;
-    float delay = sqrtf(dx * dx + dy * dy);
-    float rel_phase = 4 * pi / lambda * (delay - dx * s0);
-    float c = cosf(rel_phase);
-    float out_s2 = rs01 * rs01 + ris * ris + 2 * rs01 * ris * c;
-    float out_p2 = rp01 * rp01 + rip * rip + 2 * rp01 * rip * c;
-    return (out_s2 + out_p2) / 2;
+    return fresnel_ar_terms(theta0, lambda, d1, n0, n1, n2).r;
 }
 
 // raytracing.cl:107-120; `lambda * 1e-3` is a double product in the source
