@@ -450,6 +450,61 @@ __device__ __forceinline__ void hull_row_span(const HullRows& h, int AA, int aa_
     *b = floorshr(floorshr(sh, aa_shift), TILE_SHIFT);
 }
 
+// Quad steps whose every fragment is exactly 0 because the ghost texture
+// lookup reads only its zero border (rays that pass the aperture stop outside
+// the texture's extent: rasterizing.cl:484-485 samples the ghost with border
+// colour 0).  Such a step bins no pair; the image is unchanged, since adding
+// an exact 0 leaves a pixel's sum as it was.
+//
+// Why the fragments are exactly 0.  With one sample per pixel the shading
+// point is the covered sample itself, so it lies in the quad (interior or
+// boundary).  For a strictly convex quad the mean value coordinates of such a
+// point are non-negative and sum to 1, so an interpolated texel coordinate
+// lies between the corners' minimum and maximum; with integer vertices less
+// than CULL_EXTENT samples apart the edge terms (cross and dot products of
+// integer offsets) are exact in float32, and the rounding of the reciprocal
+// square roots and reciprocals leaves a negative weight mass far below 1e-2
+// (checked over random and sliver-shaped quads against a float32 emulation
+// of barycentric_fast with one-ulp errors: at most 5e-4).  A margin of one
+// texel plus the corners' spread therefore keeps every interpolated
+// coordinate beyond the border: at or past size (texels size, size + 1 of
+// the framed copy) or below -2 (texels -2, -1).  The lookup then blends four
+// zero texels with finite weights, which is +0 when the other coordinate is
+// finite; the remaining factors are finite (smoothstep of anything, maxima
+// with 0 of sums of values bounded by 1e30 or NaN), so the fragment is +0.
+static constexpr long CULL_EXTENT = 4096;
+static constexpr float CULL_BOUND = 1.0e6f;  // texel coordinates taken as finite
+
+__device__ __forceinline__ bool strictly_convex(const int* vx, const int* vy) {
+    int sign = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int j = (i + 1) & 3, k = (i + 2) & 3;
+        const long c = ((long)vx[j] - vx[i]) * ((long)vy[k] - vy[j]) -
+                       ((long)vy[j] - vy[i]) * ((long)vx[k] - vx[j]);
+        const int s = c > 0 ? 1 : (c < 0 ? -1 : 0);
+        if (s == 0 || (sign != 0 && s != sign)) return false;
+        sign = s;
+    }
+    return true;
+}
+
+// Every interpolation of the four texel coordinates t (with the margin above)
+// lies past the texture's edge [0, size) into its zero border.
+__device__ __forceinline__ bool past_border(const float* t, float size) {
+    const float lo = fminf(fminf(t[0], t[1]), fminf(t[2], t[3]));
+    const float hi = fmaxf(fmaxf(t[0], t[1]), fmaxf(t[2], t[3]));
+    const float spread = hi - lo;
+    return lo >= size + 1.0f + spread || hi <= -3.0f - spread;
+}
+
+__device__ __forceinline__ bool bounded(const float* t) {
+    bool ok = true;
+#pragma unroll
+    for (int j = 0; j < 4; j++) ok = ok && fabsf(t[j]) <= CULL_BOUND;  // false for NaN
+    return ok;
+}
+
 // Per (quad, step): integer vertices, edge biases, tile rectangle, pair
 // count, and the blended shading attributes.
 __global__ __launch_bounds__(256) void k_setup(const float4* __restrict__ vtx,
@@ -482,6 +537,7 @@ __global__ __launch_bounds__(256) void k_setup(const float4* __restrict__ vtx,
         const float blend = tab.x;
         int vx[4], vy[4];
         long vbase[4];
+        float tu[4], tv[4];  // ghost texel coordinates of the corners
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const long base = l.ray0 + (long)qv[j] * G.W + wl;
@@ -494,6 +550,9 @@ __global__ __launch_bounds__(256) void k_setup(const float4* __restrict__ vtx,
             vy[j] = rf_to_int(my * (float)G.AA);
             rec.vx[j] = vx[j];
             rec.vy[j] = vy[j];
+            // the corner's u, v exactly as the shading attributes blend them
+            tu[j] = (a0.z + (b0.z - a0.z) * blend) * G.gku + G.gcu;
+            tv[j] = (a0.w + (b0.w - a0.w) * blend) * G.gkv + G.gcv;
         }
         const bool collinear = edge_i(vx[0], vy[0], vx[1], vy[1], vx[2], vy[2]) == 0 &&
                                edge_i(vx[0], vy[0], vx[2], vy[2], vx[3], vy[3]) == 0;
@@ -517,6 +576,25 @@ __global__ __launch_bounds__(256) void k_setup(const float4* __restrict__ vtx,
             long sy1 = max(max(vy[0], vy[1]), max(vy[2], vy[3]));
             long x0 = floorshr(sx0, G.aa_shift), x1 = floorshr(sx1, G.aa_shift);
             long y0 = floorshr(sy0, G.aa_shift), y1 = floorshr(sy1, G.aa_shift);
+            // every fragment samples the ghost's zero border: no pairs
+            bool zero = false;
+            if (G.AA == 1 && sx1 - sx0 < CULL_EXTENT && sy1 - sy0 < CULL_EXTENT &&
+                bounded(tu) && bounded(tv) && (past_border(tu, G.gwf) || past_border(tv, G.ghf)) &&
+                strictly_convex(vx, vy)) {
+                zero = true;
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const long base = vbase[j];
+                    const float4 a1 = vtx[2 * base + 1];
+                    const float4 b1 = base + 1 < G.NV ? vtx[2 * (base + 1) + 1] : make_float4(0, 0, 0, 0);
+                    const float rr = a1.x + (b1.x - a1.x) * blend;
+                    const float rf = a1.y + (b1.y - a1.y) * blend;
+                    const float in = a1.z + (b1.z - a1.z) * blend;
+                    // NaN passes (the maxima with 0 drop it), infinities do not
+                    zero = zero && !(fabsf(rr) > 1e30f) && !(fabsf(rf) > 1e30f) && !(fabsf(in) > 1e30f);
+                }
+            }
+            if (zero) x1 = x0 - 1;  // an empty rectangle
             // bins the binner lists this quad in (rasterizing.cl:449-453)
             const int sbx0 = wadd(rf_to_int(b.x * G.st), G.width / 2);
             const int sby0 = wadd(rf_to_int(b.y * G.st), G.height / 2);
