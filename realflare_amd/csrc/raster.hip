@@ -450,13 +450,28 @@ __device__ __forceinline__ void hull_row_span(const HullRows& h, int AA, int aa_
     *b = floorshr(floorshr(sh, aa_shift), TILE_SHIFT);
 }
 
-// Quad steps whose every fragment is exactly 0 because the ghost texture
-// lookup reads only its zero border (rays that pass the aperture stop outside
-// the texture's extent: rasterizing.cl:484-485 samples the ghost with border
-// colour 0).  Such a step bins no pair; the image is unchanged, since adding
-// an exact 0 leaves a pixel's sum as it was.
+// Quad steps whose every fragment is exactly 0 bin no pair; the image is
+// unchanged, since adding an exact 0 leaves a pixel's sum as it was.  A
+// fragment is ghost * smoothstep(rrel) * max(intensity, 0) * max(coating, 0)
+// (rasterizing.cl:471-506) with each attribute a weighted sum of the four
+// corners'.  Two cases are recognized:
 //
-// Why the fragments are exactly 0.  With one sample per pixel the shading
+// (1) A corner whose ray died (NaN reflectance; the reference leaves its
+// position and attributes undefined, rasterizing.cl:347-351).  The weighted
+// reflectance is then NaN at every pixel, whatever the weights, and its
+// maximum with 0 is 0, so the fragment is 0 as long as the other factors are
+// finite: corner attributes bounded (below) and finite weights.  The weights
+// of barycentric_fast are finite unless their sum vanishes to the last bit at
+// a sample, a rounding coincidence at which the fragment is NaN and poisons
+// its pixel, here and in the reference alike, at positions that rounding
+// (and, in the reference, the undefined corner) decides; such a pixel is not
+// reproduced by any evaluation other than the reference's own.
+//
+// (2) The ghost texture lookup reads only its zero border (rays that pass the
+// aperture stop outside the texture's extent: rasterizing.cl:484-485 samples
+// the ghost with border colour 0).
+//
+// Why the fragments of (2) are exactly 0.  With one sample per pixel the shading
 // point is the covered sample itself, so it lies in the quad (interior or
 // boundary).  For a strictly convex quad the mean value coordinates of such a
 // point are non-negative and sum to 1, so an interpolated texel coordinate
@@ -576,12 +591,10 @@ __global__ __launch_bounds__(256) void k_setup(const float4* __restrict__ vtx,
             long sy1 = max(max(vy[0], vy[1]), max(vy[2], vy[3]));
             long x0 = floorshr(sx0, G.aa_shift), x1 = floorshr(sx1, G.aa_shift);
             long y0 = floorshr(sy0, G.aa_shift), y1 = floorshr(sy1, G.aa_shift);
-            // every fragment samples the ghost's zero border: no pairs
+            // every fragment is exactly 0: no pairs
             bool zero = false;
-            if (G.AA == 1 && sx1 - sx0 < CULL_EXTENT && sy1 - sy0 < CULL_EXTENT &&
-                bounded(tu) && bounded(tv) && (past_border(tu, G.gwf) || past_border(tv, G.ghf)) &&
-                strictly_convex(vx, vy)) {
-                zero = true;
+            if (bounded(tu) && bounded(tv)) {
+                bool finite = true, dead = false;
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
                     const long base = vbase[j];
@@ -591,8 +604,13 @@ __global__ __launch_bounds__(256) void k_setup(const float4* __restrict__ vtx,
                     const float rf = a1.y + (b1.y - a1.y) * blend;
                     const float in = a1.z + (b1.z - a1.z) * blend;
                     // NaN passes (the maxima with 0 drop it), infinities do not
-                    zero = zero && !(fabsf(rr) > 1e30f) && !(fabsf(rf) > 1e30f) && !(fabsf(in) > 1e30f);
+                    finite = finite && !(fabsf(rr) > 1e30f) && !(fabsf(rf) > 1e30f) && !(fabsf(in) > 1e30f);
+                    dead = dead || isnan(rf);
                 }
+                zero = finite && (dead || (G.AA == 1 && sx1 - sx0 < CULL_EXTENT &&
+                                           sy1 - sy0 < CULL_EXTENT &&
+                                           (past_border(tu, G.gwf) || past_border(tv, G.ghf)) &&
+                                           strictly_convex(vx, vy)));
             }
             if (zero) x1 = x0 - 1;  // an empty rectangle
             // bins the binner lists this quad in (rasterizing.cl:449-453)
