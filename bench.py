@@ -220,6 +220,10 @@ def adjoint_timing(wl, steps: int) -> dict:
                                           / times['forward_ms'])
         out[f'K{K}'] = times
     out['rays'] = 66 * 8 * 64 * 64
+    # GhostTrace's default checkpoint spacing (TraceSpec checkpoint_every=2)
+    d = out['K2']
+    out['default'] = {'K': 2, 'fused_over_forward': d['fused_over_forward'],
+                      'gradient_over_forward': d['gradient_over_forward']}
     return out
 
 
