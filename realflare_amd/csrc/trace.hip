@@ -37,6 +37,32 @@ __device__ float dispersion(float lambda, const float* c) {
     return sqrtf(1 + d0 + d1 + d2);
 }
 
+// Per workgroup (one wavelength): s_n[i] = refractive index of element i
+// (raytracing.cl:301-306), s_o[2 i] and s_o[2 i + 1] = the refraction ratio
+// n1 / n2 at element i for a ray travelling towards -z (n1 of element i - 1)
+// and towards +z (n1 of element i + 1), 1 outside the lens (raytracing.cl:
+// 295-298, 70-86).  The same divisions as a per-ray evaluation, so the same
+// values.  Every thread of the workgroup must call it.
+__device__ __forceinline__ void lens_indices(const rf_lens_element* __restrict__ lens, int L,
+                                             float wavelength, float* s_n, float* s_o) {
+#pragma clang fp contract(off)
+    const bool disperse = !isnan(lens[0].coefficients[0]);
+    const int t = threadIdx.x;
+    if (t < L) {
+        const rf_lens_element& e = lens[t];
+        float n = e.ior;
+        if (disperse && n > 1) n = dispersion(wavelength, e.coefficients);
+        s_n[t] = n;
+    }
+    __syncthreads();
+    if (t < L) {
+        const float n2 = s_n[t];
+        s_o[2 * t] = (t - 1 >= 0 ? s_n[t - 1] : 1.0f) / n2;
+        s_o[2 * t + 1] = (t + 1 < L ? s_n[t + 1] : 1.0f) / n2;
+    }
+    __syncthreads();
+}
+
 struct Hit {
     f3 pos, normal;
     float incident;
@@ -94,7 +120,8 @@ struct RayOut {
 template <int OUT>
 __device__ __forceinline__ RayOut trace_ray(const rf_lens_element* __restrict__ lens, int L,
                                             int aperture, int b1, int b2, float wavelength,
-                                            const float* s_n, int g, float grid_length,
+                                            const float* s_n, const float* s_o, int g,
+                                            float grid_length,
                                             const float* direction, float coating_min_ior,
                                             const float* __restrict__ coating_lambda, int ray_id,
                                             bool active, void* __restrict__ inters, long idx) {
@@ -151,12 +178,20 @@ __device__ __forceinline__ RayOut trace_ray(const rf_lens_element* __restrict__ 
                     apt_x = h.pos.x / height;
                     apt_y = h.pos.y / height;
                 } else {
-                    rrel = fmaxf(rrel, sqrtf(h.pos.x * h.pos.x + h.pos.y * h.pos.y) / height);
+                    // rrel = max(rrel, |xy| / height) (raytracing.cl:280).  The
+                    // square root and the quotient are skipped where |xy|^2 is
+                    // below (rrel height)^2 (1 - 2^-20): then the rounded
+                    // quotient is below rrel, whatever the roundings, and the
+                    // maximum is rrel.
+                    const float s2 = h.pos.x * h.pos.x + h.pos.y * h.pos.y;
+                    const double bound = (double)rrel * (double)height;
+                    if (!(height > 0 && (double)s2 < bound * bound * (1.0 - 0x1p-20)))
+                        rrel = fmaxf(rrel, sqrtf(s2) / height);
                     if (!is_sensor) {
-                        int n_index = (dir.z < 0) ? lens_id - 1 : lens_id + 1;
-                        float n1 = (0 <= n_index && n_index < L) ? s_n[n_index] : 1.0f;
-                        float n2 = s_n[lens_id];
                         if (do_reflect) {
+                            int n_index = (dir.z < 0) ? lens_id - 1 : lens_id + 1;
+                            float n1 = (0 <= n_index && n_index < L) ? s_n[n_index] : 1.0f;
+                            float n2 = s_n[lens_id];
                             // reflect, raytracing.cl:96-102
                             dir = add3(dir, scl3(h.normal, 2.0f * dot3(h.normal, scl3(dir, -1.0f))));
                             float theta = (float)((double)h.incident + 1e-9);
@@ -171,8 +206,8 @@ __device__ __forceinline__ RayOut trace_ray(const rf_lens_element* __restrict__ 
                             }
                             if (r > 0) refl *= r;
                         } else {
-                            // refract, raytracing.cl:70-86
-                            float o = n1 / n2;
+                            // refract, raytracing.cl:70-86; o = n1 / n2
+                            float o = s_o[2 * lens_id + (dir.z < 0 ? 0 : 1)];
                             float cost = dot3(scl3(dir, -1.0f), h.normal);
                             float sint2 = (o * o) * (1 - cost * cost);
                             f3 t = add3(scl3(dir, o),
@@ -219,24 +254,18 @@ void k_trace(const rf_lens_element* __restrict__ lens,
                                                const float* __restrict__ coating_lambda) {
 #pragma clang fp contract(off)
     __shared__ float s_n[RF_LMAX];
+    __shared__ float s_o[2 * RF_LMAX];
     const int L = prm.lens_count;
     const int w = blockIdx.y, p = blockIdx.z;
     const int g = prm.grid_count, R = g * g, W = prm.wavelength_count;
     const float wavelength = (float)wavelengths[w];
-    const bool disperse = !isnan(lens[0].coefficients[0]);
-    if ((int)threadIdx.x < L) {
-        const rf_lens_element& e = lens[threadIdx.x];
-        float n = e.ior;
-        if (disperse && n > 1) n = dispersion(wavelength, e.coefficients);
-        s_n[threadIdx.x] = n;
-    }
-    __syncthreads();
+    lens_indices(lens, L, wavelength, s_n, s_o);
 
     const int ray_id = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = ray_id < R;
     const long idx = ((long)p * W + w) * R + ray_id;
     const RayOut r = trace_ray<OUT>(lens, L, prm.aperture_index, paths[2 * p], paths[2 * p + 1],
-                                    wavelength, s_n, g, prm.grid_length, prm.direction,
+                                    wavelength, s_n, s_o, g, prm.grid_length, prm.direction,
                                     prm.coating_min_ior, coating_lambda, ray_id, active, inters, idx);
     if (!active) return;
 
@@ -325,6 +354,7 @@ __global__ __launch_bounds__(BATCH_THREADS) void k_trace_batch(
     float* __restrict__ planes) {
 #pragma clang fp contract(off)
     __shared__ float s_n[RF_LMAX];
+    __shared__ float s_o[2 * RF_LMAX];
     __shared__ uint64_t s_chunk;
     const uint64_t total = chunk_offsets[nseg];
     int cur_lens = -1, cur_w = -1;
@@ -345,15 +375,8 @@ __global__ __launch_bounds__(BATCH_THREADS) void k_trace_batch(
         const rf_lens_element* lens = lenses + first;
         const float wavelength = (float)wavelengths[w];
         if (sg.lens != cur_lens || w != cur_w) {
-            // the previous chunk's lanes are done with s_n (barrier above)
-            const bool disperse = !isnan(lens[0].coefficients[0]);
-            if ((int)threadIdx.x < L) {
-                const rf_lens_element& e = lens[threadIdx.x];
-                float n = e.ior;
-                if (disperse && n > 1) n = dispersion(wavelength, e.coefficients);
-                s_n[threadIdx.x] = n;
-            }
-            __syncthreads();
+            // the previous chunk's lanes are done with s_n, s_o (barrier above)
+            lens_indices(lens, L, wavelength, s_n, s_o);
             cur_lens = sg.lens;
             cur_w = w;
         }
@@ -362,7 +385,7 @@ __global__ __launch_bounds__(BATCH_THREADS) void k_trace_batch(
         for (int k = 0; k < RPL; k++) {
             const int ray_id = (blk * RPL + k) * BATCH_THREADS + threadIdx.x;
             const bool active = ray_id < R;
-            const RayOut r = trace_ray<OUT_PLANES>(lens, L, aperture, b1, b2, wavelength, s_n, g,
+            const RayOut r = trace_ray<OUT_PLANES>(lens, L, aperture, b1, b2, wavelength, s_n, s_o, g,
                                                    sg.grid_length, sg.direction, coating_min_ior,
                                                    nullptr, ray_id, active, nullptr, 0);
             if (active) {
