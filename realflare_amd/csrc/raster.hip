@@ -908,11 +908,6 @@ __global__ __launch_bounds__(256) void k_expand_heavy(const Setup* __restrict__ 
     }
 }
 
-__global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ p, long n, uint32_t v) {
-    const long i = (long)blockIdx.x * 256 + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
 // Tile ranges of the sorted pairs: each thread looks at four consecutive keys
 // (one 16-byte load; the key array is 16-byte aligned) and the key after them.
 __global__ __launch_bounds__(256) void k_ranges(const uint32_t* __restrict__ keys,
@@ -1195,6 +1190,64 @@ __global__ __launch_bounds__(256) void k_block_cost(const uint32_t* __restrict__
         }
     keys[i] = 0xffffu - (uint32_t)min(total >> 2, (uint64_t)0xffffu);  // 16-bit key: 2 passes
     vals[i] = (uint32_t)i;
+}
+
+// The super-block order in one workgroup, for runs of at most
+// BLOCK_ORDER_MAX super-blocks: a counting sort on a logarithmic cost bucket
+// (16 buckets per doubling of the pair count, the most expensive first).  The
+// order only schedules the rasterizer's workgroups (every tile's pixels are
+// computed alike in any order), so super-blocks of one bucket may come in any
+// order.  Writes a permutation of the run's super-blocks to vals.
+static constexpr int BLOCK_ORDER_MAX = 8192;
+static constexpr int BLOCK_ORDER_THREADS = 1024;
+static constexpr int BLOCK_BUCKETS = 256;
+
+__global__ __launch_bounds__(BLOCK_ORDER_THREADS) void k_block_order(
+    const uint32_t* __restrict__ tstart, const uint32_t* __restrict__ tend, Geom G,
+    uint32_t* __restrict__ vals, Header* __restrict__ h) {
+    __shared__ uint32_t s_bucket[BLOCK_ORDER_MAX];
+    __shared__ uint32_t s_count[BLOCK_BUCKETS];
+    const long nsb = super_blocks(G);
+    const int n = (int)(nsb * G.frames);
+    const int t = threadIdx.x;
+    if (t == 0) h->blocks = (uint64_t)n;
+    if (t < BLOCK_BUCKETS) s_count[t] = 0;
+    __syncthreads();
+    const int SB = G.sb, nsbx = (G.ntx + SB - 1) / SB;
+    for (int i = t; i < n; i += BLOCK_ORDER_THREADS) {
+        const long frame = i / nsb, sb = i % nsb;
+        const int x0 = (int)(sb % nsbx) * SB, y0 = (int)(sb / nsbx) * SB;
+        uint64_t total = 0;
+        for (int ty = y0; ty < min(y0 + SB, G.nty); ty++)
+            for (int tx = x0; tx < min(x0 + SB, G.ntx); tx++) {
+                const long tl = frame * G.ntx * G.nty + (long)ty * G.ntx + tx;
+                total += tend[tl] - tstart[tl];
+            }
+        const int level = min(BLOCK_BUCKETS - 1, (int)(16.0f * __log2f((float)total + 1.0f)));
+        const uint32_t b = (uint32_t)(BLOCK_BUCKETS - 1 - level);
+        s_bucket[i] = b;
+        atomicAdd(&s_count[b], 1u);
+    }
+    __syncthreads();
+    if (t == 0) {  // 256 buckets: an exclusive scan by one lane
+        uint32_t run = 0;
+        for (int b = 0; b < BLOCK_BUCKETS; b++) {
+            const uint32_t c = s_count[b];
+            s_count[b] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    for (int i = t; i < n; i += BLOCK_ORDER_THREADS) vals[atomicAdd(&s_count[s_bucket[i]], 1u)] = (uint32_t)i;
+}
+
+// tstart and tend cleared by one launch
+__global__ __launch_bounds__(256) void k_clear_ranges(uint32_t* __restrict__ a, uint32_t* __restrict__ b,
+                                                      long n) {
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+        a[i] = 0u;
+        b[i] = 0u;
+    }
 }
 
 // fragment_shader (rasterizing.cl:471-506) for one covered pixel of one quad
@@ -1610,20 +1663,24 @@ static int bin_impl(const float4* vtx, const float4* bounds, const float* spectr
     while ((1L << bits) < ntiles) bits++;
     rc = radix_sort_pairs(r.keys, r.vals, &r.h->n_sort, cap, bits, r.radix_ws, s);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(ntiles, 256)), dim3(256), 0, s, r.tstart, ntiles,
-                       0u);
-    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(ntiles, 256)), dim3(256), 0, s, r.tend, ntiles, 0u);
+    hipLaunchKernelGGL(k_clear_ranges, dim3(blocks_for(ntiles, 256)), dim3(256), 0, s, r.tstart,
+                       r.tend, ntiles);
     if (cap > 0) {
         hipLaunchKernelGGL(k_ranges, dim3(blocks_for(((long)cap + 3) / 4, 256)), dim3(256), 0, s,
                            r.keys, r.h, r.tstart, r.tend);
     }
     // the rasterizer's super-block order: largest pair count first (stable)
     const long nblk = super_blocks(G) * G.frames;
-    hipLaunchKernelGGL(k_block_cost, dim3(blocks_for(nblk, 256)), dim3(256), 0, s, r.tstart, r.tend,
-                       G, r.order_keys, r.order_vals, r.h);
-    rc = radix_sort_pairs(r.order_keys, r.order_vals, &r.h->blocks, (uint64_t)nblk, 16, r.radix_ws,
-                          s);
-    if (rc) return rc;
+    if (nblk <= BLOCK_ORDER_MAX) {
+        hipLaunchKernelGGL(k_block_order, dim3(1), dim3(BLOCK_ORDER_THREADS), 0, s, r.tstart, r.tend,
+                           G, r.order_vals, r.h);
+    } else {
+        hipLaunchKernelGGL(k_block_cost, dim3(blocks_for(nblk, 256)), dim3(256), 0, s, r.tstart,
+                           r.tend, G, r.order_keys, r.order_vals, r.h);
+        rc = radix_sort_pairs(r.order_keys, r.order_vals, &r.h->blocks, (uint64_t)nblk, 16,
+                              r.radix_ws, s);
+        if (rc) return rc;
+    }
     RF_LAUNCH_CHECK("binner");
     return RF_OK;
 }
