@@ -241,9 +241,9 @@ __device__ __forceinline__ RayOut trace_ray(const rf_lens_element* __restrict__ 
     return o;
 }
 
-// Six waves per SIMD for the ray records and planes (80 VGPRs: one register
-// fewer than the unconstrained allocation, which left room for five); the
-// intersection records need more registers and keep the default.
+// At least six waves per SIMD for the ray records and planes (an upper bound
+// of 80 VGPRs; the kernel takes 52 and runs eight waves per SIMD); the
+// intersection records keep the default bound.
 template <int OUT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OUT == OUT_INTERSECTIONS ? 1 : 6)))
 void k_trace(const rf_lens_element* __restrict__ lens,
