@@ -352,6 +352,20 @@ struct LensF {
     float wavelength;
 };
 
+// s_n holds a workgroup's refractive indices in [0, L) and, from RF_LMAX on,
+// the refraction ratio n1 / n2 of element i for a ray travelling towards -z
+// (s_n[RF_LMAX + 2 i]) and towards +z (s_n[RF_LMAX + 2 i + 1]), as the
+// tracer's per-workgroup table (the same divisions, so the same values).
+// Fills the ratios; the indices must be written and visible to every thread.
+__device__ __forceinline__ void refraction_ratios(float* s_n, int L, int t) {
+#pragma clang fp contract(off)
+    if (t < L) {
+        const float n2 = s_n[t];
+        s_n[RF_LMAX + 2 * t] = (t - 1 >= 0 ? s_n[t - 1] : 1.0f) / n2;
+        s_n[RF_LMAX + 2 * t + 1] = (t + 1 < L ? s_n[t + 1] : 1.0f) / n2;
+    }
+}
+
 // One event forward (trace.hip semantics).  Returns false when the ray dies.
 __device__ __forceinline__ bool event_fwd(const LensF& Lf, const float* s_n, int L, int stop, int el,
                                           bool reflect, St& s) {
@@ -380,7 +394,7 @@ __device__ __forceinline__ bool event_fwd(const LensF& Lf, const float* s_n, int
         if (r > 0) s.refl *= r;
         return true;
     }
-    const float o = n1 / n2;
+    const float o = s_n[RF_LMAX + 2 * el + (dir_in.z < 0 ? 0 : 1)];  // n1 / n2
     const float cost = dot3(scl3(dir_in, -1.0f), I.n);
     const float sint2 = (o * o) * (1 - cost * cost);
     f3 t = add3(scl3(dir_in, o), scl3(I.n, o * cost - sqrtf(fabsf(1 - sint2))));
@@ -652,7 +666,7 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_trace_adjoint(
     const float* __restrict__ gx, const float* __restrict__ gy, const float* __restrict__ gr,
     float* __restrict__ scratch, float* __restrict__ partial, float* __restrict__ fwd_out) {
 #pragma clang fp contract(off)
-    __shared__ float s_n[RF_LMAX];
+    __shared__ float s_n[3 * RF_LMAX];  // indices, then refraction ratios
     extern __shared__ float s_lacc[];  // [ADJ_LDS_ROWS L][ADJ_THREADS]
     const int L = prm.lens_count;
     const int w = blockIdx.y, p = blockIdx.z;
@@ -667,6 +681,8 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_trace_adjoint(
     }
     Grad G{s_lacc, 0.0f, 0.0f};
     if (MODE != ADJ_FORWARD) grad_clear(G, L);
+    __syncthreads();
+    refraction_ratios(s_n, L, threadIdx.x);
     __syncthreads();
 
     const long N = (long)prm.path_count * W * R;
@@ -830,7 +846,7 @@ __device__ __forceinline__ float fit_term(float r, float delta, float* d) {
 template <int K>
 __global__ __launch_bounds__(ADJ_THREADS) void k_trace_fit(FitArgs a) {
 #pragma clang fp contract(off)
-    __shared__ float s_n[RF_LMAX];
+    __shared__ float s_n[3 * RF_LMAX];  // indices, then refraction ratios
     __shared__ uint64_t s_chunk;
     extern __shared__ float s_lacc[];  // [ADJ_LDS_ROWS L][ADJ_THREADS]
     const int lane = threadIdx.x;
@@ -866,6 +882,8 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_trace_fit(FitArgs a) {
         }
         Grad G{s_lacc, 0.0f, 0.0f};
         grad_clear(G, L);
+        __syncthreads();
+        refraction_ratios(s_n, L, lane);
         __syncthreads();
         LensF Lf{lens, a.coatings + first, a.cmin, wavelength};
         Path P;
