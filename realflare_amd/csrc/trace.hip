@@ -112,6 +112,156 @@ struct RayOut {
     float apt_x, apt_y, rrel, refl;
 };
 
+// The state of one ray of the `raytrace` kernel between two elements.
+struct RayState {
+    f3 pos, dir;
+    float apt_x, apt_y, rrel, refl;
+    bool alive, died;
+};
+
+// init_ray (raytracing.cl:178-215) of grid ray `ray_id` of a g x g grid.
+__device__ __forceinline__ RayState init_state(const rf_lens_element* __restrict__ lens, int g,
+                                               float grid_length, const float* direction,
+                                               int ray_id, bool active) {
+#pragma clang fp contract(off)
+    RayState s;
+    int y = (int)truncf((float)ray_id / (float)g);
+    int x = ray_id - y * g;
+    float px = (float)((double)grid_length * ((double)((float)x / (float)(g - 1)) - 0.5));
+    float py = (float)((double)grid_length * (0.5 - (double)((float)y / (float)(g - 1))));
+    Hit h = intersect<false>(mk3(px, py, 1.0f), mk3(0, 0, -1.0f), lens[0].radius, lens[0].center);
+    f3 d = norm3(mk3(-direction[0], -direction[1], -direction[2]));
+    s.pos = sub3(h.pos, d);
+    s.dir = d;
+    s.apt_x = 0.0f;
+    s.apt_y = 0.0f;
+    s.rrel = 0.0f;
+    s.refl = 1.0f;
+    s.alive = active;
+    s.died = false;
+    return s;
+}
+
+// Where the raytrace loop stands: the next element, the bounce count and the
+// direction of travel through the element list, the next intersection record.
+struct Cursor {
+    int lens_id, step, delta, inter_id;
+};
+
+// The events of the raytrace loop (raytracing.cl:234-352) from the cursor on,
+// while the element index stays in [0, limit) (limit = L: until the ray
+// leaves the lens), for ghost path (b1, b2); refractive indices of the
+// wavelength in s_n, refraction ratios in s_o.  A wave stops as soon as a
+// ballot shows no live lane: the remaining events would change nothing.
+// Callers keep the path, wavelength, lens and cursor wave-uniform.
+template <int OUT>
+__device__ __forceinline__ void run_events(RayState& s, Cursor& c, int limit,
+                                           const rf_lens_element* __restrict__ lens, int L,
+                                           int aperture, int b1, int b2, float wavelength,
+                                           const float* s_n, const float* s_o,
+                                           float coating_min_ior,
+                                           const float* __restrict__ coating_lambda,
+                                           void* __restrict__ inters, long idx) {
+#pragma clang fp contract(off)
+    const int inter_count = 3 * L - 1;
+    while (c.lens_id >= 0 && c.lens_id < limit) {
+        const int lens_id = c.lens_id;
+        const float radius = lens[lens_id].radius;
+        const float center = lens[lens_id].center;
+        const float height = lens[lens_id].height;
+        const int coating = lens[lens_id].coating;
+        const float coating_f = coating_lambda ? coating_lambda[lens_id] : (float)coating;
+        const bool is_stop = lens_id == aperture;
+        const bool is_sensor = lens_id == L - 1;
+        const bool do_reflect = !is_stop && !is_sensor &&
+                                ((c.step == 0 && lens_id == b1) || (c.step == 1 && lens_id == b2));
+        if (s.alive) {
+            // refraction events never read the incidence angle, except that
+            // the intersection records store it
+            Hit h = (do_reflect || OUT == OUT_INTERSECTIONS)
+                        ? intersect<true>(s.pos, s.dir, radius, center)
+                        : intersect<false>(s.pos, s.dir, radius, center);
+            if (!h.hit) {
+                s.alive = false;
+                s.died = true;
+            } else {
+                if (OUT == OUT_INTERSECTIONS && c.inter_id < inter_count) {
+                    float4* rec = (float4*)inters + 3 * (idx * inter_count + c.inter_id);
+                    rec[0] = make_float4(h.pos.x, h.pos.y, h.pos.z, 0.0f);
+                    rec[1] = make_float4(h.normal.x, h.normal.y, h.normal.z, 0.0f);
+                    rec[2] = make_float4(h.incident, __int_as_float(1), 0.0f, 0.0f);
+                }
+                s.pos = h.pos;
+                if (is_stop) {
+                    s.apt_x = h.pos.x / height;
+                    s.apt_y = h.pos.y / height;
+                } else {
+                    // rrel = max(rrel, |xy| / height) (raytracing.cl:280).  The
+                    // square root and the quotient are skipped where |xy|^2 is
+                    // below (rrel height)^2 (1 - 2^-20): then the rounded
+                    // quotient is below rrel, whatever the roundings, and the
+                    // maximum is rrel.
+                    const float s2 = h.pos.x * h.pos.x + h.pos.y * h.pos.y;
+                    const double bound = (double)s.rrel * (double)height;
+                    if (!(height > 0 && (double)s2 < bound * bound * (1.0 - 0x1p-20)))
+                        s.rrel = fmaxf(s.rrel, sqrtf(s2) / height);
+                    if (!is_sensor) {
+                        if (do_reflect) {
+                            int n_index = (s.dir.z < 0) ? lens_id - 1 : lens_id + 1;
+                            float n1 = (0 <= n_index && n_index < L) ? s_n[n_index] : 1.0f;
+                            float n2 = s_n[lens_id];
+                            // reflect, raytracing.cl:96-102
+                            s.dir = add3(s.dir,
+                                         scl3(h.normal, 2.0f * dot3(h.normal, scl3(s.dir, -1.0f))));
+                            float theta = (float)((double)h.incident + 1e-9);
+                            float r;
+                            if (coating_f > 0) {
+                                float lambda_c = coating_f;
+                                float nc = fmaxf(sqrtf(n1 * n2), coating_min_ior);
+                                float d1 = lambda_c / (4.0f * nc);
+                                r = fresnel_ar(theta, wavelength, d1, n1, nc, n2);
+                            } else {
+                                r = fresnel(theta, n1, n2);
+                            }
+                            if (r > 0) s.refl *= r;
+                        } else {
+                            // refract, raytracing.cl:70-86; o = n1 / n2
+                            float o = s_o[2 * lens_id + (s.dir.z < 0 ? 0 : 1)];
+                            float cost = dot3(scl3(s.dir, -1.0f), h.normal);
+                            float sint2 = (o * o) * (1 - cost * cost);
+                            f3 t = add3(scl3(s.dir, o),
+                                        scl3(h.normal, o * cost - sqrtf(fabsf(1 - sint2))));
+                            s.dir = scl3(t, (float)(sint2 < 1));
+                            if (s.dir.z == 0) {
+                                s.alive = false;
+                                s.died = true;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        if (do_reflect) {
+            c.step++;
+            c.delta = -c.delta;
+        }
+        c.inter_id++;
+        c.lens_id += c.delta;
+        if (!__any(s.alive)) break;
+    }
+}
+
+__device__ __forceinline__ RayOut ray_out(const RayState& s) {
+    RayOut o;
+    o.pos = s.pos;
+    o.dir = s.dir;
+    o.apt_x = s.apt_x;
+    o.apt_y = s.apt_y;
+    o.rrel = s.rrel;
+    o.refl = s.died ? NAN : s.refl;
+    return o;
+}
+
 // One ray of the `raytrace` kernel (raytracing.cl:178-359): grid ray `ray_id`
 // of a g x g grid through lens[0..L), ghost path (b1, b2), refractive indices
 // of the ray's wavelength in s_n.  Returns the final state; reflectance is NaN
@@ -125,125 +275,29 @@ __device__ __forceinline__ RayOut trace_ray(const rf_lens_element* __restrict__ 
                                             const float* direction, float coating_min_ior,
                                             const float* __restrict__ coating_lambda, int ray_id,
                                             bool active, void* __restrict__ inters, long idx) {
-#pragma clang fp contract(off)
-    // init_ray, raytracing.cl:178-215
-    f3 pos, dir;
-    {
-        int y = (int)truncf((float)ray_id / (float)g);
-        int x = ray_id - y * g;
-        float px = (float)((double)grid_length *
-                           ((double)((float)x / (float)(g - 1)) - 0.5));
-        float py = (float)((double)grid_length *
-                           (0.5 - (double)((float)y / (float)(g - 1))));
-        Hit h = intersect<false>(mk3(px, py, 1.0f), mk3(0, 0, -1.0f), lens[0].radius,
-                                 lens[0].center);
-        f3 d = norm3(mk3(-direction[0], -direction[1], -direction[2]));
-        pos = sub3(h.pos, d);
-        dir = d;
-    }
-    float apt_x = 0.0f, apt_y = 0.0f, rrel = 0.0f, refl = 1.0f;
-
-    bool alive = active, died = false;
-    int lens_id = 0, step = 0, delta = 1, inter_id = 0;
-    const int inter_count = 3 * L - 1;
-
-    while (lens_id >= 0 && lens_id < L) {
-        const float radius = lens[lens_id].radius;
-        const float center = lens[lens_id].center;
-        const float height = lens[lens_id].height;
-        const int coating = lens[lens_id].coating;
-        const float coating_f = coating_lambda ? coating_lambda[lens_id] : (float)coating;
-        const bool is_stop = lens_id == aperture;
-        const bool is_sensor = lens_id == L - 1;
-        const bool do_reflect =
-            !is_stop && !is_sensor && ((step == 0 && lens_id == b1) || (step == 1 && lens_id == b2));
-        if (alive) {
-            // refraction events never read the incidence angle, except that
-            // the intersection records store it
-            Hit h = (do_reflect || OUT == OUT_INTERSECTIONS)
-                        ? intersect<true>(pos, dir, radius, center)
-                        : intersect<false>(pos, dir, radius, center);
-            if (!h.hit) {
-                alive = false;
-                died = true;
-            } else {
-                if (OUT == OUT_INTERSECTIONS && inter_id < inter_count) {
-                    float4* rec = (float4*)inters + 3 * (idx * inter_count + inter_id);
-                    rec[0] = make_float4(h.pos.x, h.pos.y, h.pos.z, 0.0f);
-                    rec[1] = make_float4(h.normal.x, h.normal.y, h.normal.z, 0.0f);
-                    rec[2] = make_float4(h.incident, __int_as_float(1), 0.0f, 0.0f);
-                }
-                pos = h.pos;
-                if (is_stop) {
-                    apt_x = h.pos.x / height;
-                    apt_y = h.pos.y / height;
-                } else {
-                    // rrel = max(rrel, |xy| / height) (raytracing.cl:280).  The
-                    // square root and the quotient are skipped where |xy|^2 is
-                    // below (rrel height)^2 (1 - 2^-20): then the rounded
-                    // quotient is below rrel, whatever the roundings, and the
-                    // maximum is rrel.
-                    const float s2 = h.pos.x * h.pos.x + h.pos.y * h.pos.y;
-                    const double bound = (double)rrel * (double)height;
-                    if (!(height > 0 && (double)s2 < bound * bound * (1.0 - 0x1p-20)))
-                        rrel = fmaxf(rrel, sqrtf(s2) / height);
-                    if (!is_sensor) {
-                        if (do_reflect) {
-                            int n_index = (dir.z < 0) ? lens_id - 1 : lens_id + 1;
-                            float n1 = (0 <= n_index && n_index < L) ? s_n[n_index] : 1.0f;
-                            float n2 = s_n[lens_id];
-                            // reflect, raytracing.cl:96-102
-                            dir = add3(dir, scl3(h.normal, 2.0f * dot3(h.normal, scl3(dir, -1.0f))));
-                            float theta = (float)((double)h.incident + 1e-9);
-                            float r;
-                            if (coating_f > 0) {
-                                float lambda_c = coating_f;
-                                float nc = fmaxf(sqrtf(n1 * n2), coating_min_ior);
-                                float d1 = lambda_c / (4.0f * nc);
-                                r = fresnel_ar(theta, wavelength, d1, n1, nc, n2);
-                            } else {
-                                r = fresnel(theta, n1, n2);
-                            }
-                            if (r > 0) refl *= r;
-                        } else {
-                            // refract, raytracing.cl:70-86; o = n1 / n2
-                            float o = s_o[2 * lens_id + (dir.z < 0 ? 0 : 1)];
-                            float cost = dot3(scl3(dir, -1.0f), h.normal);
-                            float sint2 = (o * o) * (1 - cost * cost);
-                            f3 t = add3(scl3(dir, o),
-                                        scl3(h.normal, o * cost - sqrtf(fabsf(1 - sint2))));
-                            dir = scl3(t, (float)(sint2 < 1));
-                            if (dir.z == 0) {
-                                alive = false;
-                                died = true;
-                            }
-                        }
-                    }
-                }
-            }
-        }
-        if (do_reflect) {
-            step++;
-            delta = -delta;
-        }
-        inter_id++;
-        lens_id += delta;
-        if (!__any(alive)) break;
-    }
-    if (died) refl = NAN;
-    RayOut o;
-    o.pos = pos;
-    o.dir = dir;
-    o.apt_x = apt_x;
-    o.apt_y = apt_y;
-    o.rrel = rrel;
-    o.refl = refl;
-    return o;
+    RayState s = init_state(lens, g, grid_length, direction, ray_id, active);
+    Cursor c{0, 0, 1, 0};
+    run_events<OUT>(s, c, L, lens, L, aperture, b1, b2, wavelength, s_n, s_o, coating_min_ior,
+                    coating_lambda, inters, idx);
+    return ray_out(s);
 }
 
+// Ghosts per workgroup of k_trace (ray records and planes).  Every ghost path
+// (b1, b2) runs the same events as the straight path through elements
+// 0 .. b1 - 1: up to its first reflection at b1 it refracts through the same
+// elements as every other ghost of the same grid ray and wavelength.  A
+// workgroup traces a group of consecutive ghosts: it advances one
+// straight-through state from element to element and continues a copy of it
+// from each ghost's b1 along that ghost's own path, so the shared prefix is
+// traced once per group instead of once per ghost.  The events are the same
+// operations on the same values, so the outputs are the same bits.  Paths in
+// increasing b1 (all_pair_paths' order) share the most; a ghost whose b1 lies
+// behind the straight state restarts it from init_ray.
+static constexpr int TRACE_GHOST_GROUP = 4;
+
 // At least six waves per SIMD for the ray records and planes (an upper bound
-// of 80 VGPRs; the kernel takes 52 and runs eight waves per SIMD); the
-// intersection records keep the default bound.
+// of 80 VGPRs); the intersection records (one ghost per workgroup: they
+// record every event) keep the default bound.
 template <int OUT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OUT == OUT_INTERSECTIONS ? 1 : 6)))
 void k_trace(const rf_lens_element* __restrict__ lens,
@@ -256,33 +310,52 @@ void k_trace(const rf_lens_element* __restrict__ lens,
     __shared__ float s_n[RF_LMAX];
     __shared__ float s_o[2 * RF_LMAX];
     const int L = prm.lens_count;
-    const int w = blockIdx.y, p = blockIdx.z;
-    const int g = prm.grid_count, R = g * g, W = prm.wavelength_count;
+    const int w = blockIdx.y;
+    const int g = prm.grid_count, R = g * g, W = prm.wavelength_count, P = prm.path_count;
     const float wavelength = (float)wavelengths[w];
     lens_indices(lens, L, wavelength, s_n, s_o);
 
     const int ray_id = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = ray_id < R;
-    const long idx = ((long)p * W + w) * R + ray_id;
-    const RayOut r = trace_ray<OUT>(lens, L, prm.aperture_index, paths[2 * p], paths[2 * p + 1],
-                                    wavelength, s_n, s_o, g, prm.grid_length, prm.direction,
-                                    prm.coating_min_ior, coating_lambda, ray_id, active, inters, idx);
-    if (!active) return;
-
-    if (OUT == OUT_PLANES) {
-        const long N = (long)prm.path_count * W * R;
-        float* o = (float*)out;
-        o[idx] = r.pos.x;
-        o[N + idx] = r.pos.y;
-        o[2 * N + idx] = r.apt_x;
-        o[3 * N + idx] = r.apt_y;
-        o[4 * N + idx] = r.rrel;
-        o[5 * N + idx] = r.refl;
-    } else {
-        float4* o = (float4*)out + 3 * idx;
-        o[0] = make_float4(r.pos.x, r.pos.y, r.pos.z, 0.0f);
-        o[1] = make_float4(r.dir.x, r.dir.y, r.dir.z, 0.0f);
-        o[2] = make_float4(r.apt_x, r.apt_y, r.rrel, r.refl);
+    const int group = OUT == OUT_INTERSECTIONS ? 1 : TRACE_GHOST_GROUP;
+    const int p0 = blockIdx.z * group, p1 = min(P, p0 + group);
+    // the straight-through state: elements [0, bc.lens_id) traced
+    RayState base = init_state(lens, g, prm.grid_length, prm.direction, ray_id, active);
+    Cursor bc{0, 0, 1, 0};
+    for (int p = p0; p < p1; p++) {
+        const int b1 = paths[2 * p], b2 = paths[2 * p + 1];
+        // the ghost's events equal the straight path's below this element
+        const int start = (b1 >= 0 && b1 < L) ? b1 : L;
+        if (start < bc.lens_id) {
+            base = init_state(lens, g, prm.grid_length, prm.direction, ray_id, active);
+            bc = Cursor{0, 0, 1, 0};
+        }
+        const long idx = ((long)p * W + w) * R + ray_id;
+        // (intersection records: one ghost per workgroup, so the prefix's
+        // records are this ghost's)
+        run_events<OUT>(base, bc, start, lens, L, prm.aperture_index, -1, -1, wavelength, s_n, s_o,
+                        prm.coating_min_ior, coating_lambda, inters, idx);
+        RayState st = base;
+        Cursor c = bc;
+        run_events<OUT>(st, c, L, lens, L, prm.aperture_index, b1, b2, wavelength, s_n, s_o,
+                        prm.coating_min_ior, coating_lambda, inters, idx);
+        if (!active) continue;
+        const RayOut r = ray_out(st);
+        if (OUT == OUT_PLANES) {
+            const long N = (long)P * W * R;
+            float* o = (float*)out;
+            o[idx] = r.pos.x;
+            o[N + idx] = r.pos.y;
+            o[2 * N + idx] = r.apt_x;
+            o[3 * N + idx] = r.apt_y;
+            o[4 * N + idx] = r.rrel;
+            o[5 * N + idx] = r.refl;
+        } else {
+            float4* o = (float4*)out + 3 * idx;
+            o[0] = make_float4(r.pos.x, r.pos.y, r.pos.z, 0.0f);
+            o[1] = make_float4(r.dir.x, r.dir.y, r.dir.z, 0.0f);
+            o[2] = make_float4(r.apt_x, r.apt_y, r.rrel, r.refl);
+        }
     }
 }
 
@@ -314,7 +387,8 @@ static int launch_trace(const rf_lens_element* lens, const int32_t* paths,
     if (rc) return rc;
     if (prm->path_count == 0 || prm->wavelength_count == 0) return RF_OK;
     const int R = prm->grid_count * prm->grid_count;
-    dim3 grid((R + 255) / 256, prm->wavelength_count, prm->path_count);
+    const int group = OUT == OUT_INTERSECTIONS ? 1 : TRACE_GHOST_GROUP;
+    dim3 grid((R + 255) / 256, prm->wavelength_count, (prm->path_count + group - 1) / group);
     hipLaunchKernelGGL(k_trace<OUT>, grid, dim3(256), 0, (hipStream_t)stream, lens, paths,
                        wavelengths, *prm, out, inters, coating_lambda);
     RF_LAUNCH_CHECK("k_trace");
