@@ -570,68 +570,83 @@ __device__ __forceinline__ RayInit init_ray_state(const rf_lens_element* __restr
     return r;
 }
 
-// Forward pass of one ray over its E events, starting from s.  The current
-// segment's input states are kept in st[] (on return, the last segment's);
-// segment start states go to checkpoint slot = segment of the ray's record
-// (sidx in a [slot][field][N] array): every segment's when store_all, else
-// all but the last.  Returns whether the ray reaches the sensor; s is the
-// final state.
+// Forward pass of events [k0, P.E) of one ray, starting from s (the state
+// before event k0; `alive` whether the ray still travels there).  Segments of
+// K events start at k0: the current segment's input states are kept in st[]
+// (on return, the last segment's); segment start states go to checkpoint slot
+// = segment of the ray's record (sidx in a [slot][field][N] array): every
+// segment's when store_all, else all but the last.  Returns whether the ray
+// reaches the sensor; s is the final state.
 template <int K>
-__device__ __forceinline__ bool forward_ray(const LensF& Lf, const float* s_n, const Path& P, St& s,
-                                            St (&st)[K], float* scratch, long N, long sidx,
-                                            bool active, bool store_all) {
-    bool alive = active;
-    const int nseg = (P.E + K - 1) / K;
+__device__ __forceinline__ bool forward_span(const LensF& Lf, const float* s_n, const Path& P,
+                                             int k0, St& s, bool alive, St (&st)[K],
+                                             float* scratch, long N, long sidx, bool active,
+                                             bool store_all) {
+    const int nseg = (P.E - k0 + K - 1) / K;
     for (int seg = 0; seg < nseg; seg++) {
-        const int k0 = seg * K;
+        const int kb = k0 + seg * K;
         if (active && (store_all || seg + 1 < nseg)) st_store(scratch, N, sidx, seg, s);
 #pragma unroll
         for (int j = 0; j < K; j++) {
-            if (k0 + j < P.E) {
+            if (kb + j < P.E) {
                 st[j] = s;
                 if (alive)
-                    alive = event_fwd(Lf, s_n, P.L, P.stop, event_element(P, k0 + j),
-                                      event_reflects(P, k0 + j), s);
+                    alive = event_fwd(Lf, s_n, P.L, P.stop, event_element(P, kb + j),
+                                      event_reflects(P, kb + j), s);
             }
         }
     }
     return alive;
 }
 
-// Reverse pass of a ray that reaches the sensor with final state `fin`:
-// (posb, dirb, reflb) is the adjoint of the final state.  Segments are walked
-// backwards and rebuilt from their checkpoints (the last one too unless its
-// states are still in st[]); the init_ray step closes the chain.
 template <int K>
-__device__ __forceinline__ void reverse_ray(const LensF& Lf, const float* s_n, Grad& G,
-                                            const Path& P, const RayInit& ri, const St& fin,
-                                            St (&st)[K], const float* scratch, long N, long sidx,
-                                            bool last_in_registers, f3 posb, f3 dirb, float reflb) {
+__device__ __forceinline__ bool forward_ray(const LensF& Lf, const float* s_n, const Path& P, St& s,
+                                            St (&st)[K], float* scratch, long N, long sidx,
+                                            bool active, bool store_all) {
+    return forward_span<K>(Lf, s_n, P, 0, s, active, st, scratch, N, sidx, active, store_all);
+}
+
+// Reverse pass of events [k0, P.E) of a ray that reaches the sensor with final
+// state `fin`: (posb, dirb, reflb) is the adjoint of the final state on entry,
+// (posb, dirb) that of the state before event k0 on return.  Segments are
+// walked backwards and rebuilt from their checkpoints (the last one too unless
+// its states are still in st[]).
+template <int K>
+__device__ __forceinline__ void reverse_span(const LensF& Lf, const float* s_n, Grad& G,
+                                             const Path& P, int k0, const St& fin, St (&st)[K],
+                                             const float* scratch, long N, long sidx,
+                                             bool last_in_registers, f3& posb, f3& dirb,
+                                             float reflb) {
 #pragma clang fp contract(off)
-    const int nseg = (P.E + K - 1) / K;
+    const int nseg = (P.E - k0 + K - 1) / K;
     f3 hit = fin.pos;  // hit point of the event being reversed
     for (int seg = nseg - 1; seg >= 0; seg--) {
-        const int k0 = seg * K;
+        const int kb = k0 + seg * K;
         if (!last_in_registers || seg + 1 < nseg) {
             st[0] = st_load(scratch, N, sidx, seg);
 #pragma unroll
             for (int j = 1; j < K; j++) {
-                if (k0 + j < P.E) {
+                if (kb + j < P.E) {
                     st[j] = st[j - 1];
-                    event_fwd(Lf, s_n, P.L, P.stop, event_element(P, k0 + j - 1),
-                              event_reflects(P, k0 + j - 1), st[j]);
+                    event_fwd(Lf, s_n, P.L, P.stop, event_element(P, kb + j - 1),
+                              event_reflects(P, kb + j - 1), st[j]);
                 }
             }
         }
 #pragma unroll
         for (int j = K - 1; j >= 0; j--) {
-            if (k0 + j < P.E) {
-                event_rev(Lf, s_n, G, P, k0 + j, st[j], hit, posb, dirb, reflb);
+            if (kb + j < P.E) {
+                event_rev(Lf, s_n, G, P, kb + j, st[j], hit, posb, dirb, reflb);
                 hit = st[j].pos;
             }
         }
     }
-    // init_ray: pos0 = I0.p - d0 with the vertical probe ray; d0 is constant
+}
+
+// init_ray (pos0 = I0.p - d0 with the vertical probe ray; d0 is constant) in
+// reverse, given the adjoint posb of the first state.
+__device__ __forceinline__ void init_rev(const LensF& Lf, Grad& G, int L, const RayInit& ri,
+                                         f3 posb) {
     if (ri.I0.hit) {
         float Rb = 0.0f, Cb = 0.0f;
         f3 gpb = mk3(0, 0, 0), ddb = mk3(0, 0, 0);
@@ -640,8 +655,20 @@ __device__ __forceinline__ void reverse_ray(const LensF& Lf, const float* s_n, G
         hit_bwd(h0, ri.gpt, ri.down, ri.I0.p, e0.radius, e0.center, posb, mk3(0, 0, 0), 0.0f, gpb,
                 ddb, Rb, Cb);
         acc(G, 0, Rb);
-        acc(G, P.L, Cb);
+        acc(G, L, Cb);
     }
+}
+
+// Reverse pass of a whole ray that reaches the sensor (events [0, P.E), then
+// init_ray).
+template <int K>
+__device__ __forceinline__ void reverse_ray(const LensF& Lf, const float* s_n, Grad& G,
+                                            const Path& P, const RayInit& ri, const St& fin,
+                                            St (&st)[K], const float* scratch, long N, long sidx,
+                                            bool last_in_registers, f3 posb, f3 dirb, float reflb) {
+    reverse_span<K>(Lf, s_n, G, P, 0, fin, st, scratch, N, sidx, last_in_registers, posb, dirb,
+                    reflb);
+    init_rev(Lf, G, P.L, ri, posb);
 }
 
 // What one launch of k_trace_adjoint does.
@@ -651,26 +678,61 @@ enum AdjMode {
     ADJ_REVERSE = 2,  // reverse only, from what an ADJ_FORWARD launch left in the scratch
 };
 
+// Ghosts per workgroup of k_trace_adjoint.  A ghost path (b1, b2) runs the
+// straight path's events through elements 0 .. b1 - 1 (the same refractions
+// for every ghost of one grid ray and wavelength) before its first
+// reflection.  The workgroup traces that straight chain once for its group of
+// ghosts, keeping every state S_j (before element j) in the straight tape,
+// and each ghost from S_b1 on.  In reverse, the loss is a sum over ghosts, so
+// the straight chain's adjoint at S_j is the sum of the adjoints that the
+// ghosts branching off at or after j hand down; the chain is reversed once,
+// from its far end to S_0, taking each ghost's adjoint where the ghost branches
+// off (ghosts in decreasing b1).  The straight events are the same operations
+// on the same values as a per-ghost trace, so the outputs are the same bits.
+static constexpr int ADJ_GHOST_GROUP = 4;
+
+__device__ __forceinline__ Path ghost_path(const int* __restrict__ paths, int p, int L, int stop) {
+    Path P;
+    P.b1 = paths[2 * p];
+    P.b2 = paths[2 * p + 1];
+    P.L = L;
+    P.stop = stop;
+    P.E = P.b1 >= 0 ? L + 2 * (P.b1 - P.b2) : L;
+    return P;
+}
+
+// The first event of ghost path P that differs from the straight path's
+// (its first reflection); a path without reflection follows the straight
+// path throughout; irregular paths share nothing.
+__device__ __forceinline__ int branch_event(const Path& P) {
+    if (P.b1 < 0) return P.E;
+    if (P.b1 < P.L && P.b2 >= 0 && P.b2 < P.b1) return P.b1;
+    return 0;
+}
+
 // Partials (floats) are laid out [parameter][workgroup], parameters ordered
 // [radius L][centre L][coating L][n_eff L] (n_eff for the workgroup's wavelength).
 //
 // K events form a segment.  The forward pass keeps the current segment's input
 // states in registers (st[], indexed by unrolled loops) and stores segment
-// start states in the scratch (slot = segment; the final state, reflectance
-// NaN for a ray that does not reach the sensor, in slot nseg); the reverse
-// pass rebuilds each segment from its checkpoint, walking segments backwards.
+// start states in the scratch (per ghost ray, slot = segment counted from the
+// ghost's branch event; the final state, reflectance NaN for a ray that does
+// not reach the sensor, in the slot after the last segment); the reverse pass
+// rebuilds each segment from its checkpoint, walking segments backwards.
 template <int K, int MODE>
 __global__ __launch_bounds__(ADJ_THREADS) void k_trace_adjoint(
     const rf_lens_element* __restrict__ lens, const float* __restrict__ coating,
     const int* __restrict__ paths, const int* __restrict__ wavelengths, rf_trace_params prm,
     const float* __restrict__ gx, const float* __restrict__ gy, const float* __restrict__ gr,
-    float* __restrict__ scratch, float* __restrict__ partial, float* __restrict__ fwd_out) {
+    float* __restrict__ scratch, float* __restrict__ straight, float* __restrict__ partial,
+    float* __restrict__ fwd_out) {
 #pragma clang fp contract(off)
     __shared__ float s_n[3 * RF_LMAX];  // indices, then refraction ratios
-    extern __shared__ float s_lacc[];  // [ADJ_LDS_ROWS L][ADJ_THREADS]
+    __shared__ float s_coat[RF_LMAX];   // the group's coating adjoints (wave sums)
+    extern __shared__ float s_lacc[];   // [ADJ_LDS_ROWS L][ADJ_THREADS]
     const int L = prm.lens_count;
-    const int w = blockIdx.y, p = blockIdx.z;
-    const int g = prm.grid_count, R = g * g, W = prm.wavelength_count;
+    const int w = blockIdx.y, grp = blockIdx.z;
+    const int g = prm.grid_count, R = g * g, W = prm.wavelength_count, NP = prm.path_count;
     const float wavelength = (float)wavelengths[w];
     const bool disperse = !isnan(lens[0].coefficients[0]);
     if ((int)threadIdx.x < L) {
@@ -678,6 +740,7 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_trace_adjoint(
         float n = e.ior;
         if (disperse && n > 1) n = dispersion_f(wavelength, e.coefficients);
         s_n[threadIdx.x] = n;
+        s_coat[threadIdx.x] = 0.0f;
     }
     Grad G{s_lacc, 0.0f, 0.0f};
     if (MODE != ADJ_FORWARD) grad_clear(G, L);
@@ -685,50 +748,144 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_trace_adjoint(
     refraction_ratios(s_n, L, threadIdx.x);
     __syncthreads();
 
-    const long N = (long)prm.path_count * W * R;
+    const long N = (long)NP * W * R;  // ghost-ray records
+    const long NS = (long)gridDim.z * W * R;  // straight-chain records
     const int ray_id = blockIdx.x * ADJ_THREADS + threadIdx.x;
     const bool active = ray_id < R;
-    const long idx = ((long)p * W + w) * R + (active ? ray_id : 0);
+    const int rid = active ? ray_id : 0;
+    const long sidx = ((long)grp * W + w) * R + rid;
     LensF Lf{lens, coating, prm.coating_min_ior, wavelength};
-    Path P;
-    P.b1 = paths[2 * p];
-    P.b2 = paths[2 * p + 1];
-    P.L = L;
-    P.stop = prm.aperture_index;
-    P.E = P.b1 >= 0 ? L + 2 * (P.b1 - P.b2) : L;
-    const int nseg = (P.E + K - 1) / K;
-
-    const RayInit ri = init_ray_state(lens, g, prm.grid_length, prm.direction, active ? ray_id : 0);
-    St s = ri.s;
-    St st[K];
-    bool alive = active;
-    if (MODE != ADJ_REVERSE) {
-        alive = forward_ray<K>(Lf, s_n, P, s, st, scratch, N, idx, active, MODE == ADJ_FORWARD);
-        if (fwd_out && active) {
-            fwd_out[idx] = alive ? s.pos.x : NAN;
-            fwd_out[N + idx] = alive ? s.pos.y : NAN;
-            fwd_out[2 * N + idx] = alive ? s.refl : NAN;
-        }
-        if (MODE == ADJ_FORWARD) {
-            if (active) {
-                St fin = s;
-                if (!alive) fin.refl = NAN;
-                st_store(scratch, N, idx, nseg, fin);
-            }
-            return;
-        }
-    } else if (active) {
-        s = st_load(scratch, N, idx, nseg);
-        alive = !isnan(s.refl);
+    // the group's ghosts, in decreasing branch event (then increasing index):
+    // an insertion sort of (branch event, index) keys over constant indices,
+    // the order packed 4 bits a ghost
+    const int p0 = grp * ADJ_GHOST_GROUP;
+    const int np = min(ADJ_GHOST_GROUP, NP - p0);
+    int key[ADJ_GHOST_GROUP];
+    int jmax = 0;
+#pragma unroll
+    for (int i = 0; i < ADJ_GHOST_GROUP; i++) {
+        const int p = p0 + min(i, np - 1);
+        const Path P = ghost_path(paths, p, L, prm.aperture_index);
+        const int k0 = i < np ? branch_event(P) : -1;
+        jmax = max(jmax, k0);
+        key[i] = (k0 + 1) * 16 + (ADJ_GHOST_GROUP - 1 - i);  // larger first
     }
-    // reverse (rays that do not reach the sensor contribute nothing)
-    if (alive)
-        reverse_ray<K>(Lf, s_n, G, P, ri, s, st, scratch, N, idx, MODE == ADJ_FUSED,
-                       mk3(gx[idx], gy[idx], 0.0f), mk3(0, 0, 0), gr[idx]);
+#pragma unroll
+    for (int i = 1; i < ADJ_GHOST_GROUP; i++)
+#pragma unroll
+        for (int j = i; j > 0; j--)
+            if (key[j] > key[j - 1]) {
+                const int t = key[j];
+                key[j] = key[j - 1];
+                key[j - 1] = t;
+            }
+    uint32_t order = 0;
+#pragma unroll
+    for (int i = 0; i < ADJ_GHOST_GROUP; i++)
+        order |= (uint32_t)(ADJ_GHOST_GROUP - 1 - (key[i] & 15)) << (4 * i);
+    const Path Ps{-1, -1, L, L, prm.aperture_index};  // the straight path
+    St st[K];
+
+    // straight chain S_0 .. S_jmax; died = the event that ends the ray (or jmax)
+    int died = jmax;
+    if (MODE != ADJ_REVERSE) {
+        St s = init_ray_state(lens, g, prm.grid_length, prm.direction, rid).s;
+        bool alive = active;
+        for (int j = 0; j <= jmax; j++) {
+            if (active) st_store(straight, NS, sidx, j, s);
+            if (j == jmax) break;
+            if (alive) {
+                alive = event_fwd(Lf, s_n, L, prm.aperture_index, j, false, s);
+                if (!alive) died = j;
+            }
+        }
+    }
+    // adjoint of the straight state at element jcur
+    f3 Ap = mk3(0, 0, 0), Ad = mk3(0, 0, 0);
+    int jcur = jmax;
+    // straight events jcur - 1 .. j_to in reverse (lanes holding a non-zero
+    // adjoint: a ghost branching off at or after jcur reached the sensor)
+    auto straight_rev = [&](int j_to) {
+        for (int j = jcur - 1; j >= j_to; j--) {
+            const bool any = Ap.x != 0.0f || Ap.y != 0.0f || Ap.z != 0.0f || Ad.x != 0.0f ||
+                             Ad.y != 0.0f || Ad.z != 0.0f;
+            if (active && any) {
+                const St in = st_load(straight, NS, sidx, j);
+                const f3 hit = st_load(straight, NS, sidx, j + 1).pos;
+                float rb = 0.0f;
+                event_rev(Lf, s_n, G, Ps, j, in, hit, Ap, Ad, rb);
+            }
+        }
+        jcur = min(jcur, j_to);
+    };
+#pragma unroll 1
+    for (int oi = 0; oi < np; oi++) {
+        const int i = (int)((order >> (4 * oi)) & 15u);
+        const Path P = ghost_path(paths, p0 + i, L, prm.aperture_index);
+        const int k0 = branch_event(P);
+        const long idx = ((long)(p0 + i) * W + w) * R + rid;
+        const int nseg = (P.E - k0 + K - 1) / K;
+        if (MODE != ADJ_FORWARD) straight_rev(k0);
+        St s;
+        bool alive;
+        if (MODE != ADJ_REVERSE) {
+            s = st_load(straight, NS, sidx, active ? k0 : 0);
+            alive = active && died >= k0;
+            alive = forward_span<K>(Lf, s_n, P, k0, s, alive, st, scratch, N, idx, active,
+                                    MODE == ADJ_FORWARD);
+            if (fwd_out && active) {
+                fwd_out[idx] = alive ? s.pos.x : NAN;
+                fwd_out[N + idx] = alive ? s.pos.y : NAN;
+                fwd_out[2 * N + idx] = alive ? s.refl : NAN;
+            }
+            if (MODE == ADJ_FORWARD) {
+                if (active) {
+                    St fin = s;
+                    if (!alive) fin.refl = NAN;
+                    st_store(scratch, N, idx, nseg, fin);
+                }
+                continue;
+            }
+        } else {
+            alive = false;
+            if (active) {
+                s = st_load(scratch, N, idx, nseg);
+                alive = !isnan(s.refl);
+            }
+        }
+        // the ghost's events in reverse (rays that do not reach the sensor
+        // contribute nothing), then its adjoint joins the straight chain's
+        G.coat_b1 = 0.0f;
+        G.coat_b2 = 0.0f;
+        if (alive) {
+            f3 pb = mk3(gx[idx], gy[idx], 0.0f), db = mk3(0, 0, 0);
+            reverse_span<K>(Lf, s_n, G, P, k0, s, st, scratch, N, idx, MODE == ADJ_FUSED, pb, db,
+                            gr[idx]);
+            Ap = add3(Ap, pb);
+            Ad = add3(Ad, db);
+        }
+        // coating adjoints of the ghost's two reflecting elements
+        const float c1 = wave_sum(G.coat_b1), c2 = wave_sum(G.coat_b2);
+        if (threadIdx.x == 0) {
+            if (P.b1 >= 0 && P.b1 < L) s_coat[P.b1] += c1;
+            if (P.b2 >= 0 && P.b2 < L) s_coat[P.b2] += c2;
+        }
+    }
+    if (MODE == ADJ_FORWARD) return;
+    straight_rev(0);
+    {
+        const bool any = Ap.x != 0.0f || Ap.y != 0.0f || Ap.z != 0.0f;
+        if (active && any)
+            init_rev(Lf, G, L, init_ray_state(lens, g, prm.grid_length, prm.direction, rid), Ap);
+    }
+    __syncthreads();  // s_coat complete
     const long blk = ((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
     const long nblk = (long)gridDim.x * gridDim.y * gridDim.z;
     for (int i = 0; i < 4 * L; i++) {
-        const float v = grad_sum(G, P, i);
+        float v;
+        if (i < 2 * L) v = wave_sum(G.lacc[i * ADJ_THREADS + threadIdx.x]);
+        else if (i < 3 * L) v = s_coat[i - 2 * L];
+        else v = wave_sum(G.lacc[(i - L) * ADJ_THREADS + threadIdx.x]);
         if (threadIdx.x == 0) partial[i * nblk + blk] = v;
     }
 }
@@ -739,7 +896,7 @@ static constexpr int RED_THREADS = 256;
 // One workgroup per output: lane j sums the output's workgroup partials
 // j, j + 256, ... in order, then the lanes combine in a fixed tree.
 __global__ __launch_bounds__(RED_THREADS) void k_adjoint_reduce(const float* __restrict__ partial,
-                                                                 int L, int bx, int W, int P,
+                                                                 int L, int bx, int W, int P,  // P: ghost groups
                                                                  float* __restrict__ out) {
     __shared__ float s_w[RED_THREADS / 64];
     const int t = blockIdx.x;
@@ -751,7 +908,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_adjoint_reduce(const float* __r
     } else {
         const int w = (t - 3 * L) / L, i = (t - 3 * L) % L;
         const float* src = partial + (long)(3 * L + i) * nblk;
-        const long n = (long)P * bx;  // the workgroups of wavelength w: bx per ghost
+        const long n = (long)P * bx;  // the workgroups of wavelength w: bx per ghost group
         for (long j = threadIdx.x; j < n; j += RED_THREADS) {
             const long p = j / bx, x = j - p * bx;
             v += src[(p * W + w) * bx + x];
@@ -769,7 +926,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_adjoint_reduce(const float* __r
 
 typedef void (*AdjKernel)(const rf_lens_element*, const float*, const int*, const int*,
                           rf_trace_params, const float*, const float*, const float*, float*, float*,
-                          float*);
+                          float*, float*);
 
 template <int MODE>
 static AdjKernel adj_kernel_k(int K) {
@@ -1045,8 +1202,8 @@ static int max_events(const int* host_paths, int P, int L) {
 }
 
 struct AdjLayout {
-    size_t scratch, partial, total;
-    int slots, bx;
+    size_t scratch, straight, partial, total;
+    int slots, bx, groups;
 };
 
 static AdjLayout adj_layout(const rf_trace_params* prm, int K) {
@@ -1057,11 +1214,15 @@ static AdjLayout adj_layout(const rf_trace_params* prm, int K) {
     const int E = max_events(nullptr, prm->path_count, L);
     A.slots = (E + K - 1) / K + 1;  // segment starts and the final state
     A.bx = (int)((R + ADJ_THREADS - 1) / ADJ_THREADS);
+    A.groups = (prm->path_count + ADJ_GHOST_GROUP - 1) / ADJ_GHOST_GROUP;
+    const long NS = (long)A.groups * prm->wavelength_count * R;
     size_t o = 0;
     A.scratch = o;
     o += align_up((size_t)A.slots * ADJ_FIELDS * N * sizeof(float), 256);
+    A.straight = o;  // straight-chain states S_0 .. S_L per group ray
+    o += align_up((size_t)(L + 1) * ADJ_FIELDS * NS * sizeof(float), 256);
     A.partial = o;
-    o += align_up((size_t)A.bx * prm->wavelength_count * prm->path_count * 4 * L * sizeof(float), 256);
+    o += align_up((size_t)A.bx * prm->wavelength_count * A.groups * 4 * L * sizeof(float), 256);
     A.total = o;
     return A;
 }
@@ -1099,17 +1260,17 @@ static int adj_launch(int mode, const rf_lens_element* lens, const float* coatin
                       void* workspace, float* grad_out, float* forward_out, rf_stream_t stream) {
     hipStream_t s = (hipStream_t)stream;
     char* w = (char*)workspace;
-    dim3 grid(A.bx, prm->wavelength_count, prm->path_count);
+    dim3 grid(A.bx, prm->wavelength_count, A.groups);
     const size_t lds =
         mode == ADJ_FORWARD ? 0 : (size_t)ADJ_LDS_ROWS * prm->lens_count * ADJ_THREADS * sizeof(float);
     hipLaunchKernelGGL(adj_kernel(K, mode), grid, dim3(ADJ_THREADS), lds, s, lens, coating_lambda,
                        paths, wavelengths, *prm, gx, gy, gr, (float*)(w + A.scratch),
-                       (float*)(w + A.partial), forward_out);
+                       (float*)(w + A.straight), (float*)(w + A.partial), forward_out);
     if (mode != ADJ_FORWARD) {
         const int L = prm->lens_count, nout = 3 * L + prm->wavelength_count * L;
         hipLaunchKernelGGL(k_adjoint_reduce, dim3(nout), dim3(RED_THREADS), 0, s,
                            (const float*)(w + A.partial), L, A.bx, prm->wavelength_count,
-                           prm->path_count, grad_out);
+                           A.groups, grad_out);
     }
     RF_LAUNCH_CHECK("rf_trace_adjoint");
     return RF_OK;
