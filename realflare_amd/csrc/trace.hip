@@ -41,7 +41,7 @@ __device__ float dispersion(float lambda, const float* c) {
 // (raytracing.cl:301-306), s_o[2 i] and s_o[2 i + 1] = the refraction ratio
 // n1 / n2 at element i for a ray travelling towards -z (n1 of element i - 1)
 // and towards +z (n1 of element i + 1), 1 outside the lens (raytracing.cl:
-// 295-298, 70-86).  The same divisions as a per-ray evaluation, so the same
+// 295-298, 345).  The same divisions as a per-ray evaluation, so the same
 // values.  Every thread of the workgroup must call it.
 __device__ __forceinline__ void lens_indices(const rf_lens_element* __restrict__ lens, int L,
                                              float wavelength, float* s_n, float* s_o) {
@@ -148,7 +148,7 @@ struct Cursor {
     int lens_id, step, delta, inter_id;
 };
 
-// The events of the raytrace loop (raytracing.cl:234-352) from the cursor on,
+// The events of the raytrace loop (raytracing.cl:258-350) from the cursor on,
 // while the element index stays in [0, limit) (limit = L: until the ray
 // leaves the lens), for ghost path (b1, b2); refractive indices of the
 // wavelength in s_n, refraction ratios in s_o.  A wave stops as soon as a
