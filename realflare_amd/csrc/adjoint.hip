@@ -829,7 +829,8 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_trace_adjoint(
         St s;
         bool alive;
         if (MODE != ADJ_REVERSE) {
-            s = st_load(straight, NS, sidx, active ? k0 : 0);
+            s = St{mk3(0, 0, 0), mk3(0, 0, 0), 0.0f};  // lanes past the grid: unused
+            if (active) s = st_load(straight, NS, sidx, k0);
             alive = active && died >= k0;
             alive = forward_span<K>(Lf, s_n, P, k0, s, alive, st, scratch, N, idx, active,
                                     MODE == ADJ_FORWARD);
