@@ -14,7 +14,12 @@
 
 namespace rf {
 
-static constexpr int SB_MAX_CACHED = 1024;
+// Samples whose XYZ weight, wavelength scale and its reciprocal are held in
+// LDS at a time (20 KiB, so eight workgroups fit a CU); longer sample loops
+// refill the table chunk by chunk.
+static constexpr int SB_CHUNK = 1024;
+static constexpr int SB_BUFFER_WORD3 = 0x00020000;  // raw 32-bit buffer, gfx9 descriptor
+static constexpr uint32_t SB_OUT_OF_RANGE = 0x80000000u;  // past every record: the load returns 0
 
 // x / b correctly rounded, from ib = 1 / b correctly rounded (Markstein's
 // correction: q = x ib is within an ulp of x / b, the remainder x - q b is
@@ -28,22 +33,80 @@ __device__ __forceinline__ float div_rn(float x, float b, float ib) {
     return __builtin_fmaf(r, ib, q);
 }
 
+// The OpenCL 1.2 bilinear lookup with border 0 (sample_bilinear, one channel)
+// of the power spectrum, without branches: the texel coordinate is clamped to
+// [-2, size] before the conversion, and a texel outside the texture is read at
+// an offset past the buffer's size, which a raw buffer load returns as 0.  The
+// weights are the same arithmetic in the same order as sample_bilinear, so the
+// value is the same bits.  w, h < 2^24 (exact as floats) and w * h * 4 bytes
+// fits the descriptor's 31-bit size (both checked by the host), so j * w + i
+// of a texel inside does not wrap.
+__device__ __forceinline__ float spectrum_sample(__amdgpu_buffer_rsrc_t tex, int w, int h, float s,
+                                                 float t) {
+#pragma clang fp contract(off)
+    const float u = s * (float)w - 0.5f, v = t * (float)h - 0.5f;
+    const float fu = floorf(u), fv = floorf(v);
+    // rf_to_int(fu) and the clamped index agree on which texels lie inside:
+    // at or past w (or at or below -2) both texels of the axis are outside
+    // either way, and a NaN coordinate gives NaN weights either way
+    const int i0 = (int)__builtin_amdgcn_fmed3f(fu, -2.0f, (float)w);
+    const int j0 = (int)__builtin_amdgcn_fmed3f(fv, -2.0f, (float)h);
+    const int i1 = i0 + 1, j1 = j0 + 1;
+    const float a = u - fu, b = v - fv;
+    const bool ci0 = (uint32_t)i0 < (uint32_t)w, ci1 = (uint32_t)i1 < (uint32_t)w;
+    const bool cj0 = (uint32_t)j0 < (uint32_t)h, cj1 = (uint32_t)j1 < (uint32_t)h;
+    const uint32_t r0 = (uint32_t)j0 * (uint32_t)w, r1 = (uint32_t)j1 * (uint32_t)w;
+    const uint32_t o00 = ci0 && cj0 ? (r0 + (uint32_t)i0) * 4u : SB_OUT_OF_RANGE;
+    const uint32_t o10 = ci1 && cj0 ? (r0 + (uint32_t)i1) * 4u : SB_OUT_OF_RANGE;
+    const uint32_t o01 = ci0 && cj1 ? (r1 + (uint32_t)i0) * 4u : SB_OUT_OF_RANGE;
+    const uint32_t o11 = ci1 && cj1 ? (r1 + (uint32_t)i1) * 4u : SB_OUT_OF_RANGE;
+    const float t00 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(tex, o00, 0, 0));
+    const float t10 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(tex, o10, 0, 0));
+    const float t01 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(tex, o01, 0, 0));
+    const float t11 = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(tex, o11, 0, 0));
+    return (1.0f - a) * (1.0f - b) * t00 + a * (1.0f - b) * t10 + (1.0f - a) * b * t01 +
+           a * b * t11;
+}
 
-__global__ __launch_bounds__(256) void k_starburst(const float* __restrict__ spectrum, int sw, int sh,
+// kJitter: blur != 0 (the sample position is jittered by two noise values);
+// kRotate: the rotation is not exactly the identity.  With rotation 0 (the
+// default, and a weight >= 0, so that the power below is finite) the rotation
+// angle is exactly +-0: cos = 1 and sin = +-0 leave the position unchanged up
+// to the sign of a zero, which the texture coordinate 0.5 * x + 0.5 does not
+// see, so the two noise, power and trigonometric evaluations of the rotation
+// are skipped.  With blur 0 (the default) the jitter radius is exactly 0 and
+// the jitter's two noise and trigonometric evaluations are skipped as well.
+// Both skips are bit-identical to evaluating them.
+template <bool kJitter, bool kRotate>
+__global__ __launch_bounds__(256) void k_starburst(const float* __restrict__ spectrum_values, int sw, int sh,
                                                    const float* __restrict__ light, int light_n,
                                                    rf_starburst_params p, float4* __restrict__ out) {
 #pragma clang fp contract(off)
     // per sample: the X, Y, Z weights and the wavelength scale, and the
-    // scale's reciprocal (20 KiB at the cap, so eight workgroups fit a CU;
-    // the table's fourth component never reaches the output)
-    __shared__ float4 s_xyzs[SB_MAX_CACHED];
-    __shared__ float s_isc[SB_MAX_CACHED];
+    // scale's reciprocal (the table's fourth component never reaches the
+    // output)
+    __shared__ float4 s_xyzs[SB_CHUNK];
+    __shared__ float s_isc[SB_CHUNK];
     const float pi = 3.14159265358979323846f;
+    const __amdgpu_buffer_rsrc_t spectrum = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)spectrum_values, (short)0, sw * sh * (int)sizeof(float), SB_BUFFER_WORD3);
     const int samples = p.samples;
-    const bool cached = samples <= SB_MAX_CACHED;
-    if (cached) {
-        for (int t = threadIdx.x; t < samples; t += blockDim.x) {
-            const float step = (float)t / (float)samples;
+    const int w = p.width, h = p.height;
+    const int px = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int py = blockIdx.y * 16 + (threadIdx.x >> 4);
+    float nx = 2.0f * ((float)px / (float)(w - 1)) - 1.0f;
+    float ny = 2.0f * ((float)py / (float)(h - 1)) - 1.0f;
+    nx = nx - p.offset[0];
+    ny = ny - p.offset[1];
+    nx = nx / fmaxf(p.size[0], FLT_MIN);
+    ny = ny / fmaxf(p.size[1], FLT_MIN);
+    float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f;
+    const float fx = (float)px, fy = (float)py;
+    for (int t0 = 0; t0 < samples; t0 += SB_CHUNK) {
+        const int tn = min(SB_CHUNK, samples - t0);
+        if (t0 > 0) __syncthreads();  // the previous chunk has been read
+        for (int k = threadIdx.x; k < tn; k += blockDim.x) {
+            const float step = (float)(t0 + k) / (float)samples;
             const float wavelength = step * (float)(730 - 390) + 390.0f;
             const float sc = fmaxf(1 - (1 - (wavelength / 560.0f)) * 1.0f, FLT_MIN);
             float4 v;
@@ -51,78 +114,44 @@ __global__ __launch_bounds__(256) void k_starburst(const float* __restrict__ spe
             v.y = sample_bilinear(light + 1, light_n, 1, 4, step, 0.0f, false);
             v.z = sample_bilinear(light + 2, light_n, 1, 4, step, 0.0f, false);
             v.w = sc;
-            s_xyzs[t] = v;
-            s_isc[t] = 1.0f / sc;
+            s_xyzs[k] = v;
+            s_isc[k] = 1.0f / sc;
         }
         __syncthreads();
+        for (int k = 0; k < tn; ++k) {
+            const float4 xyzs = s_xyzs[k];  // X, Y, Z weights and the wavelength scale
+            const float isc = s_isc[k];
+            const float sc = xyzs.w;
+            const float seed = (float)((t0 + k) * 4);
+            float x = nx, y = ny;
+            if (kJitter) {
+                const float radius = p.blur * sqrtf(noise3(fx, fy, seed));
+                const float alpha = noise3(fx, fy, seed + 1) * 2 * pi;
+                float sa, ca;
+                sincos_reduced(alpha, &sa, &ca);
+                x += ca * radius;
+                y += sa * radius;
+            }
+            x = div_rn(x, sc, isc);
+            y = div_rn(y, sc, isc);
+            float rx = x, ry = y;
+            if (kRotate) {
+                const float sign = (noise3(fx, fy, seed + 2) > 0.5f) ? 1.0f : -1.0f;
+                const float angle =
+                    sign * p.rotation * (1.0f - powf(noise3(fx, fy, seed + 3), p.rotation_weight));
+                float sa, ca;
+                sincos_reduced(angle, &sa, &ca);
+                rx = x * ca + y * sa;
+                ry = y * ca - x * sa;
+            }
+            const float u = rx * 0.5f + 0.5f, v = ry * 0.5f + 0.5f;
+            const float fi = spectrum_sample(spectrum, sw, sh, u, v);
+            r0 += xyzs.x * fi;
+            r1 += xyzs.y * fi;
+            r2 += xyzs.z * fi;
+        }
     }
-    const int w = p.width, h = p.height;
-    const int px = blockIdx.x * 16 + (threadIdx.x & 15);
-    const int py = blockIdx.y * 16 + (threadIdx.x >> 4);
     if (px >= w || py >= h) return;
-    float nx = 2.0f * ((float)px / (float)(w - 1)) - 1.0f;
-    float ny = 2.0f * ((float)py / (float)(h - 1)) - 1.0f;
-    nx = nx - p.offset[0];
-    ny = ny - p.offset[1];
-    nx = nx / fmaxf(p.size[0], FLT_MIN);
-    ny = ny / fmaxf(p.size[1], FLT_MIN);
-    const int lambda_delta = 730 - 390;
-    float r0 = 0.0f, r1 = 0.0f, r2 = 0.0f;
-    const float fx = (float)px, fy = (float)py;
-    // With rotation 0 (the default, and a weight >= 0, so that the power
-    // below is finite) the rotation angle is exactly +-0: cos = 1 and
-    // sin = +-0 leave the position unchanged up to the sign of a zero, which
-    // the texture coordinate 0.5 * x + 0.5 does not see, so the two noise,
-    // power and trigonometric evaluations of the rotation are skipped.  With
-    // blur 0 (the default) the jitter radius is exactly 0 and the jitter's two
-    // noise and trigonometric evaluations are skipped as well.  Both skips
-    // are bit-identical to evaluating them.
-    const bool jitter = p.blur != 0.0f;
-    const bool rotate = !(p.rotation == 0.0f && p.rotation_weight >= 0.0f);
-    for (int t = 0; t < samples; ++t) {
-        const float step = (float)t / (float)samples;
-        float4 xyzs;  // X, Y, Z weights and the wavelength scale
-        float isc;
-        if (cached) {
-            xyzs = s_xyzs[t];
-            isc = s_isc[t];
-        } else {
-            const float wavelength = step * (float)lambda_delta + 390.0f;
-            xyzs.w = fmaxf(1 - (1 - (wavelength / 560.0f)) * 1.0f, FLT_MIN);
-            isc = 1.0f / xyzs.w;
-            xyzs.x = sample_bilinear(light + 0, light_n, 1, 4, step, 0.0f, false);
-            xyzs.y = sample_bilinear(light + 1, light_n, 1, 4, step, 0.0f, false);
-            xyzs.z = sample_bilinear(light + 2, light_n, 1, 4, step, 0.0f, false);
-        }
-        const float sc = xyzs.w;
-        const float seed = (float)(t * 4);
-        float x = nx, y = ny;
-        if (jitter) {
-            const float radius = p.blur * sqrtf(noise3(fx, fy, seed));
-            const float alpha = noise3(fx, fy, seed + 1) * 2 * pi;
-            float sa, ca;
-            sincos_reduced(alpha, &sa, &ca);
-            x += ca * radius;
-            y += sa * radius;
-        }
-        x = div_rn(x, sc, isc);
-        y = div_rn(y, sc, isc);
-        float rx = x, ry = y;
-        if (rotate) {
-            const float sign = (noise3(fx, fy, seed + 2) > 0.5f) ? 1.0f : -1.0f;
-            const float angle =
-                sign * p.rotation * (1.0f - powf(noise3(fx, fy, seed + 3), p.rotation_weight));
-            float sa, ca;
-            sincos_reduced(angle, &sa, &ca);
-            rx = x * ca + y * sa;
-            ry = y * ca - x * sa;
-        }
-        const float u = rx * 0.5f + 0.5f, v = ry * 0.5f + 0.5f;
-        const float fi = sample_bilinear(spectrum, sw, sh, 1, u, v, false);
-        r0 += xyzs.x * fi;
-        r1 += xyzs.y * fi;
-        r2 += xyzs.z * fi;
-    }
     r0 /= (float)samples;
     r1 /= (float)samples;
     r2 /= (float)samples;
@@ -152,10 +181,19 @@ extern "C" int rf_starburst(const float* spectrum, int spectrum_width, int spect
     if (prm->width < 1 || prm->height < 1 || prm->samples < 1 || spectrum_width < 1 ||
         spectrum_height < 1 || spectrum_count < 1)
         return set_error(RF_ERR_INVALID, "rf_starburst: empty size or samples < 1");
+    if ((long)spectrum_width * spectrum_height > (1L << 28) || spectrum_width >= (1 << 24) ||
+        spectrum_height >= (1 << 24))
+        return set_error(RF_ERR_INVALID,
+                         "rf_starburst: power spectrum of %dx%d exceeds 2^28 values or a side of 2^24",
+                         spectrum_width, spectrum_height);
+    const bool jitter = prm->blur != 0.0f;
+    const bool rotate = !(prm->rotation == 0.0f && prm->rotation_weight >= 0.0f);
+    decltype(&k_starburst<false, false>) kern = k_starburst<false, false>;
+    if (jitter) kern = rotate ? k_starburst<true, true> : k_starburst<true, false>;
+    else if (rotate) kern = k_starburst<false, true>;
     dim3 grid((prm->width + 15) / 16, (prm->height + 15) / 16);
-    hipLaunchKernelGGL(k_starburst, grid, dim3(256), 0, (hipStream_t)stream, spectrum,
-                       spectrum_width, spectrum_height, light_spectrum, spectrum_count, *prm,
-                       (float4*)image);
+    hipLaunchKernelGGL(kern, grid, dim3(256), 0, (hipStream_t)stream, spectrum, spectrum_width,
+                       spectrum_height, light_spectrum, spectrum_count, *prm, (float4*)image);
     RF_LAUNCH_CHECK("rf_starburst");
     return RF_OK;
 }
