@@ -224,6 +224,41 @@ def test_starburst_matches_oracle(jitter, oracle_lib):
     assert err <= (1e-3 if jitter else 1e-5)
 
 
+@pytest.mark.parametrize('blur, rotation', [(0.0, 0.0), (2.0, 0.0), (1.0, 3.0)])
+def test_starburst_long_sample_loop_matches_oracle(blur, rotation, oracle_lib):
+    """More samples than the kernel's per-sample table holds (1024), so the
+    table is refilled mid-loop; a non-square power spectrum and a light far
+    off centre with a small size, so that many lookups fall outside the
+    spectrum (border 0) on both axes."""
+    from realflare_amd import native
+    from realflare_amd.api.color import light_spectrum
+    from realflare_amd.api.data import Flare
+    from realflare_amd.api.tasks.starburst import starburst_params
+
+    sw, sh = 120, 80
+    ap = oracle_lib.aperture_shape((sw, sh), (0.75, 0.75), 5, 0.0, 0.0, 0.05)
+    spec = oracle_lib.power_spectrum(ap.astype(np.float64), 1.0)
+    cfg = Flare.Starburst()
+    cfg.blur, cfg.rotation, cfg.rotation_weight = blur, rotation, 1.0
+    res = (37, 29)
+    prm = starburst_params(cfg, res, 2100, (0.8, -0.7), (0.5, 0.6))
+    spec_t = torch.from_numpy(spec).cuda()
+    light = torch.from_numpy(light_spectrum()).cuda()
+    out = torch.empty((res[1], res[0], 4), dtype=torch.float32, device='cuda')
+    print(f'case starburst {res} x {prm.samples} samples blur={blur} rotation={rotation}',
+          flush=True)
+    native.call('rf_starburst', native.ptr(spec_t), sw, sh, native.ptr(light), light.shape[0],
+                ctypes.byref(prm), native.ptr(out), native.stream())
+    ref = oracle_lib.starburst(res, spec, light_spectrum(), prm.samples, prm.blur, prm.rotation,
+                               prm.rotation_weight, tuple(prm.vignetting), prm.intensity,
+                               tuple(prm.offset), tuple(prm.size))
+    got = out.cpu().numpy()
+    err = rel_l1(got, ref)
+    print(f'rel L1 {err:.2e}', flush=True)
+    assert np.abs(ref).sum() > 0 and np.count_nonzero(ref[..., 1]) < ref[..., 1].size
+    assert err <= (1e-3 if blur or rotation else 1e-5)
+
+
 @pytest.mark.parametrize('blur', [0.0, 1.0])
 def test_starburst_skipped_rotation_is_exact(blur):
     """Rotation 0 skips the rotation's noise and trigonometry; a rotation of
