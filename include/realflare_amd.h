@@ -451,7 +451,8 @@ typedef struct rf_starburst_params {
 
 /*
  * Starburst image float[height][width][4].  Replaces the `starburst` kernel
- * (starburst.cl:1-87).  spectrum: rf_power_spectrum output;
+ * (starburst.cl:1-87).  spectrum: rf_power_spectrum output, at most 2^28
+ * values with sides below 2^24 (RF_ERR_INVALID otherwise);
  * light_spectrum: float[spectrum_count][4] CIE XYZ.
  */
 int rf_starburst(const float *spectrum, int spectrum_width, int spectrum_height,
