@@ -61,6 +61,29 @@ def test_raster_ghost_texture_limits_without_gpu():
     assert 'ghost texture side' in native.last_error()
 
 
+def test_starburst_spectrum_limits_without_gpu():
+    """The starburst reads the power spectrum through a raw buffer with 32-bit
+    byte offsets and clamps texel coordinates as floats: at most 2^28 values
+    and sides below 2^24, RF_ERR_INVALID beyond that (checked before any
+    launch; the pointers are never read)."""
+    from realflare_amd import native
+    from realflare_amd.api.data import Flare
+    from realflare_amd.api.tasks.starburst import starburst_params
+
+    lib = native.lib()
+    fn = lib.rf_starburst
+    fn.restype = ctypes.c_int
+    prm = starburst_params(Flare.Starburst(), (16, 16), 8, (0.0, 0.0), (1.0, 1.0))
+    buf = ctypes.create_string_buffer(64)
+    for w, h in [((1 << 14) + 1, 1 << 14), (1 << 24, 1), (1, 1 << 24)]:
+        rc = fn(buf, w, h, buf, 4, ctypes.byref(prm), buf, None)
+        assert rc == native.RF_ERR_INVALID, (w, h)
+        assert 'power spectrum' in native.last_error()
+    prm.samples = 0
+    rc = fn(buf, 4, 4, buf, 4, ctypes.byref(prm), buf, None)
+    assert rc == native.RF_ERR_INVALID and 'samples' in native.last_error()
+
+
 def test_flare_static_rejects_frames_beyond_its_scratch_without_gpu():
     """rf_flare_static's workspace is a fixed device array: a frame whose
     rf_flare workspace exceeds it even with no binning pairs is refused
