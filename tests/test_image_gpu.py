@@ -224,12 +224,13 @@ def test_starburst_matches_oracle(jitter, oracle_lib):
     assert err <= (1e-3 if jitter else 1e-5)
 
 
-@pytest.mark.parametrize('blur, rotation', [(0.0, 0.0), (2.0, 0.0), (1.0, 3.0)])
+@pytest.mark.parametrize('blur, rotation', [(0.0, 0.0), (2.0, 0.0), (0.0, 3.0), (1.0, 3.0)])
 def test_starburst_long_sample_loop_matches_oracle(blur, rotation, oracle_lib):
     """More samples than the kernel's per-sample table holds (1024), so the
     table is refilled mid-loop; a non-square power spectrum and a light far
     off centre with a small size, so that many lookups fall outside the
-    spectrum (border 0) on both axes."""
+    spectrum (border 0) on both axes; each of the four jitter / rotation
+    instantiations."""
     from realflare_amd import native
     from realflare_amd.api.color import light_spectrum
     from realflare_amd.api.data import Flare
