@@ -995,6 +995,11 @@ __device__ __forceinline__ bool cover_small(int px, int py, const Edges& er) {
     return cover_terms(w01, w12, w23, w30, w20) & 1u;
 }
 
+// Bit `lane` of a wave-uniform lane mask, for the calling lane: the mask is
+// ANDed into the execution mask by one scalar instruction instead of a 64-bit
+// shift and a test per lane.
+__device__ __forceinline__ bool lane_bit(uint64_t m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
+
 // cover_small_mask at the kAA samples of every pixel: sample s of the pixel
 // whose sample-grid origin is (px, py) lies at (px + s, py + sub_offset(kAA -
 // 1 + s)) (rasterizing.cl:627-634).  Edge k there is the row at the origin
@@ -1003,8 +1008,7 @@ __device__ __forceinline__ bool cover_small(int px, int py, const Edges& er) {
 // comparison per edge.  Returns the lanes with a covered sample and sets
 // *hits to this lane's number of covered samples.  Every lane must be active.
 template <int kAA>
-__device__ __forceinline__ uint64_t cover_small_samples(int px, int py, const Edges& er, int lane,
-                                                        int* hits) {
+__device__ __forceinline__ uint64_t cover_small_samples(int px, int py, const Edges& er, int* hits) {
     uint32_t base[5];
 #pragma unroll
     for (int k = 0; k < 5; k++) base[k] = (uint32_t)edge_row(er, k, px, py);
@@ -1021,7 +1025,7 @@ __device__ __forceinline__ uint64_t cover_small_samples(int px, int py, const Ed
         }
         const uint64_t ms = cover_terms(w[0], w[1], w[2], w[3], w[4]);
         any |= ms;
-        n += (int)((ms >> lane) & 1u);
+        n += lane_bit(ms) ? 1 : 0;
     }
     *hits = n;
     return any;
@@ -1413,7 +1417,7 @@ __global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ bounds
                 m = cover_small_mask(px, py, er) & inside_mask;
                 hits = 1;  // read by the covered lanes only
             } else {
-                m = cover_small_samples<kAA>(px, py, er, lane, &hits) & inside_mask;
+                m = cover_small_samples<kAA>(px, py, er, &hits) & inside_mask;
             }
         } else {
             bool ok = inside;
@@ -1447,7 +1451,7 @@ __global__ __launch_bounds__(64) void k_raster(const float4* __restrict__ bounds
         if (m == 0) continue;
         const uint32_t n = (uint32_t)__popcll(m);
         if (qlen + n > QCAP) flush();
-        if ((m >> lane) & 1)
+        if (lane_bit(m))
             s_q[qlen + lanes_below(m)] = (uint32_t)lane | (uint32_t)hits << 6 | nslot << 10;
         if (lane == 0) s_slot[nslot] = make_uint4((uint32_t)m, (uint32_t)(m >> 32), qlen, e);
         qlen += n;
